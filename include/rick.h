@@ -1,0 +1,157 @@
+/*
+ * librick_hip.so -- MI355X (gfx950) implementation of the RICK imaging chain:
+ * w-stacking gridding -> per-w-plane 2-D C2C FFT -> w-phase correction and
+ * plane sum.
+ *
+ * Two layers, both plain C (no HIP or torch types in any signature):
+ *
+ *  1. Drop-in entry points with the reference's own signatures and semantics
+ *     (host pointers, results accumulated into caller-zeroed buffers):
+ *     wstack(), phase_correction(), plus rick_fft_planes() as the comm-free
+ *     stand-in for the FFT stage.  A C program linked against the reference's
+ *     w-stacking.o / phase_correction.o links against this library instead.
+ *
+ *  2. A device-resident plan API (device pointers, caller's HIP stream passed
+ *     as void*) that never allocates after rick_plan_create(), so its calls can
+ *     be captured in a HIP graph and run back to back without host syncs.
+ *
+ * Grid layout everywhere: w-plane major, then v (rows, Ny), then u (Nx,
+ * contiguous), complex interleaved re/im: element (u, v, w) of a plane set is
+ * at 2*(u + v*Nx + w*Nx*Ny) (w-stacking.cu:491).  FP64 grids are double[2*n],
+ * FP32 grids float[2*n].  Images are two separate double[Nx*Ny] arrays
+ * (fourier_transform.c:209-210).
+ */
+#ifndef RICK_H
+#define RICK_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* Status codes.                                                             */
+enum {
+    RICK_OK = 0,
+    RICK_ERR_INVALID = 1,     /* bad argument (sizes, kernel, wmin == wmax ...) */
+    RICK_ERR_UNSUPPORTED = 2, /* valid for the reference, not implemented here */
+    RICK_ERR_DEVICE = 3,      /* HIP runtime error / no device                 */
+    RICK_ERR_NOMEM = 4        /* device or host allocation failed               */
+};
+
+/* Convolution kernels, the reference's compile-time GAUSS / KAISERBESSEL /
+ * GAUSS_HI_PRECISION (Makefile:66-70). */
+enum { RICK_GAUSS = 0, RICK_KAISERBESSEL = 1, RICK_GAUSS_HI_PRECISION = 2 };
+
+/* Arithmetic of the grid, FFT and phase stages. */
+enum { RICK_FP64 = 0, RICK_FP32 = 1 };
+
+/* ------------------------------------------------------------------------- */
+/* 1. Drop-in entry points.                                                   */
+
+/* Replaces wstack(), w-stacking.h:39-57 / w-stacking.cu:211-543 (C build).
+ * grid is a host array of 2*num_w_planes*grid_size_x*grid_size_y doubles and
+ * is accumulated into.  dx is used for both axes (w-stacking.cu:464-465).
+ * The convolution kernel is the one set by rick_set_kernel() (default
+ * KAISERBESSEL, the reference Makefile's default).  num_threads sizes the
+ * host-side accumulate; rank selects device rank % device_count
+ * (w-stacking.hip.cpp:269).  Rows whose w-plane index is outside
+ * [0, num_w_planes) are dropped.  Errors: see rick_last_error(). */
+void wstack(int num_w_planes, unsigned int num_points, unsigned int freq_per_chan,
+            unsigned int polarizations, double *uu, double *vv, double *ww,
+            float *vis_real, float *vis_img, float *weight, double dx, double dw,
+            int w_support, int grid_size_x, int grid_size_y, double *grid,
+            int num_threads, int rank);
+
+/* Replaces phase_correction(), w-stacking.h:70-83 / phase_correction.cu:97-328.
+ * gridss: host, 2*num_w_planes*xaxis*yaxis doubles; image_real/image_imag:
+ * host, xaxis*yaxis doubles each, accumulated into.  PHASE_ON behaviour is
+ * set by rick_set_phase() (default on, Makefile:56).  Single rank only:
+ * xaxis == xaxistot and yaxis == yaxistot.  wmax == wmin is rejected (the
+ * reference divides 0/0, phase_correction.cu:102). */
+void phase_correction(double *gridss, double *image_real, double *image_imag,
+                      int xaxis, int yaxis, int num_w_planes, int xaxistot,
+                      int yaxistot, double resolution, double wmin, double wmax,
+                      int num_threads, int rank);
+
+/* Comm-free replacement of the FFT stage: fftw_data() (fourier_transform.c:
+ * 13-93, FFTW_BACKWARD, n0 = Ny, n1 = Nx, scaled by 1/(Nx*Ny)) and cuda_fft()
+ * (cuda_fft.cu:62-199) for one rank.  Reads grid, writes gridss (host arrays,
+ * same layout; may alias).  Power-of-two axes up to 8192.  Single rank:
+ * xaxis == grid_size_x and yaxis == grid_size_y. */
+void rick_fft_planes(int num_w_planes, int grid_size_x, int grid_size_y,
+                     int xaxis, int yaxis, const double *grid, double *gridss,
+                     int rank);
+
+/* Drop-in configuration (the reference fixes these at compile time). */
+int rick_set_kernel(int kind);      /* RICK_GAUSS / _KAISERBESSEL / _GAUSS_HI_PRECISION */
+int rick_get_kernel(void);
+int rick_set_phase(int phase_on);   /* 0 = plain plane sum, as without PHASE_ON */
+int rick_get_phase(void);
+
+/* Status of the last call made on this host thread. */
+int rick_last_status(void);
+const char *rick_last_error(void);
+int rick_device_count(void);
+const char *rick_version(void);
+
+/* ------------------------------------------------------------------------- */
+/* 2. Device-resident plan API (no counterpart in the reference, which has   */
+/*    only host-pointer entry points; this is its allocation-free, stream-   */
+/*    ordered core, used by the drop-in layer above).                         */
+
+typedef struct rick_plan_s *rick_plan;
+
+typedef struct {
+    int num_w_planes;
+    int grid_size_x;        /* Nx (u, contiguous) */
+    int grid_size_y;        /* Ny (v)             */
+    int kernel;             /* RICK_GAUSS ...     */
+    int w_support;          /* odd, 1..15         */
+    double dx, dw;          /* as passed to wstack: usually 1/Nx, 1/planes */
+    int precision;          /* RICK_FP64 / RICK_FP32 */
+    unsigned int max_points;    /* capacity: rows per rick_plan_grid call */
+    /* Phase stage (phase_correction.cu:97-102). */
+    double resolution, wmin, wmax;
+    int phase_on;
+    /* Gridder tiling; 0 selects the built-in defaults. */
+    int tile_u, tile_v;
+    int rows_per_task;
+    int device;             /* HIP device ordinal the plan lives on */
+} rick_plan_desc;
+
+int rick_plan_create(rick_plan *plan, const rick_plan_desc *desc);
+int rick_plan_destroy(rick_plan plan);
+/* Device bytes held by the plan (tables + gridding workspace). */
+size_t rick_plan_device_bytes(rick_plan plan);
+/* Bytes of one grid (2*planes*Nx*Ny elements of the plan's precision). */
+size_t rick_plan_grid_bytes(rick_plan plan);
+
+/* Gridding (wstack) into a device grid.  accumulate = 0 overwrites every cell
+ * of grid (no prior clear needed); 1 adds to its contents.  num_points <=
+ * max_points.  All arrays are device pointers; stream is a hipStream_t. */
+int rick_plan_grid(rick_plan plan, unsigned int num_points, unsigned int freq_per_chan,
+                   unsigned int polarizations, const double *uu, const double *vv,
+                   const double *ww, const float *vis_real, const float *vis_img,
+                   const float *weight, void *grid, int accumulate, void *stream);
+
+/* Per-plane backward 2-D FFT scaled by 1/(Nx*Ny); in == out allowed. */
+int rick_plan_fft(rick_plan plan, const void *in, void *out, void *stream);
+
+/* Phase correction and plane sum into device images (double[Nx*Ny] each). */
+int rick_plan_phase(rick_plan plan, const void *gridss, double *image_real,
+                    double *image_imag, int accumulate, void *stream);
+
+/* Whole chain: grid (overwrite) -> FFT in place -> phase into the images
+ * (overwrite).  grid is caller scratch of rick_plan_grid_bytes(). */
+int rick_plan_image(rick_plan plan, unsigned int num_points, unsigned int freq_per_chan,
+                    unsigned int polarizations, const double *uu, const double *vv,
+                    const double *ww, const float *vis_real, const float *vis_img,
+                    const float *weight, void *grid, double *image_real,
+                    double *image_imag, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RICK_H */

@@ -1,0 +1,77 @@
+// Internal declarations shared by the gfx950 kernels and the C ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/rick.h"
+
+namespace rick {
+
+// Table oversampling of the reference (w-stacking.cu:253).
+constexpr int kOversample = 5;
+// Truncated pi literal of the reference (w-stacking.h:6).
+constexpr double kRefPi = 3.14159265359;
+// Maximum convolution support handled by the gridder.
+constexpr int kMaxSupport = 15;
+constexpr int kMaxTable = kOversample * kMaxSupport;
+
+// Geometry of one gridding problem.
+struct GridGeom {
+    int planes, nx, ny;
+    int support, kl;        // kl = (support-1)/2 (w-stacking.cu:252)
+    int kind;               // RICK_GAUSS ...
+    double dx, dw;
+    int tu, tv;             // tile size (cells)
+    int tiles_u, tiles_v;   // tiles per axis
+    uint32_t nb;            // buckets = planes*tiles_u*tiles_v
+    uint32_t rmax;          // rows per task
+    int ntable;
+};
+
+// Workspace of the binned gridder, carved out of one device allocation.
+struct GridWork {
+    uint32_t *counts;       // [nb]  rows per bucket
+    uint32_t *cursor;       // [nb]  scatter cursor (starts at the bucket offset)
+    uint32_t *offsets;      // [nb+1] exclusive scan of counts
+    uint32_t *task_off;     // [nb+1] exclusive scan of tasks per bucket
+    uint32_t *split_list;   // [split_cap] buckets handled by more than one task
+    uint32_t *meta;         // [4]: total bucket rows, tasks, split buckets
+    void *records;          // [rec_cap] sorted row records
+    uint32_t rec_cap, split_cap, task_cap;
+};
+
+size_t grid_work_bytes(const GridGeom &g, uint32_t max_points, int precision);
+void grid_work_carve(const GridGeom &g, uint32_t max_points, int precision, void *base,
+                     GridWork *w);
+
+// Launches the binned gridder.  Returns a hipError_t.
+hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
+                       const double *table_d, uint32_t num_points, uint32_t F, uint32_t P,
+                       const double *uu, const double *vv, const double *ww,
+                       const float *vr, const float *vi, const float *wt, void *grid,
+                       int accumulate, hipStream_t s);
+
+// FFT: per-plane 2-D backward transform of planes x ny x nx complex, scaled.
+struct FftPlan {
+    int planes, nx, ny;
+    const void *tw_x, *tw_y;   // device twiddle tables exp(+2 pi i m/N), N entries
+};
+// Long columns (ny above 512) use two passes; an
+// in-place transform then stages each plane through tmp (one plane).
+bool fft_needs_tmp(int precision, int ny);
+hipError_t launch_fft(const FftPlan &p, int precision, const void *in, void *out, void *tmp,
+                      double scale, hipStream_t s);
+
+// Phase correction + plane sum.
+struct PhaseGeom {
+    int planes, nx, ny;
+    double wmin, dw, dwnorm;
+    int phase_on;
+    const double *ltab, *mtab;  // device: sin(xcoord*resolution) per u / per v index
+};
+hipError_t launch_phase(const PhaseGeom &p, int precision, const void *gridss, double *re,
+                        double *im, int accumulate, hipStream_t s);
+
+}  // namespace rick

@@ -1,0 +1,125 @@
+"""Writes the golden fixtures in this directory from the reference's own code.
+
+The reference ships no tests or vectors, so the vectors here are produced by
+its own wstack() and phase_correction(), compiled from w-stacking.cu and
+phase_correction.cu by ``make -C oracle ref`` (oracle/_ref/), run with one
+thread so the atomic accumulation order is the sequential row order.  Inputs
+are seeded; each .npz holds the inputs and the reference's outputs.
+
+    python tests/golden/make_golden.py
+
+Fixtures:
+  wstack_<kernel>_s<support>.npz   kernel in gauss/kb/gausshp, support 7 and 9.
+      64 x 48 grid (non-square: dx is used for both axes, w-stacking.cu:465,
+      so rows past v = 48/64 are clipped), 4 planes, 2 channels x 2
+      polarizations, NaN real parts, rows at w = 1.0 (dropped), rows at the
+      grid edges and rows placed on table-bin edges (5*|u_dist+KL| integral).
+  phase_<on|off>_p<planes>.npz     32 x 24 planes, PHASE_ON on/off, 1 and 4 planes.
+  chain_tiny.npz                   a small synthetic observation through
+      reference wstack -> CPU FFT (FFTW absent) -> reference phase_correction.
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+sys.path.insert(0, str(ROOT))
+
+from oracle import oracle as O  # noqa: E402
+from rick_amd import synth  # noqa: E402
+
+NX, NY, PLANES, F, P = 64, 48, 4, 2, 2
+
+
+def wstack_inputs(seed: int, support: int):
+    rng = np.random.default_rng(seed)
+    kl = (support - 1) // 2
+    n_rand = 300
+    uu = rng.uniform(0.0, 1.0, n_rand)
+    vv = rng.uniform(0.0, 1.0, n_rand)
+    ww = rng.uniform(0.0, 1.0, n_rand)
+    # Edge rows: near u/v = 0 and 1, on the v clip at NY, w at the ends.
+    edge_u = np.array([0.0, 1e-9, 0.5 / NX, (kl - 0.5) / NX, 1 - 1e-9, 1 - 0.5 / NX, 0.5, 0.25])
+    edge_v = np.array([0.3, 0.7, (NY - 0.5) / NX, (NY + 1.5) / NX, 0.1, 1e-9, (kl + 0.3) / NX,
+                       (NY - kl) / NX])
+    edge_w = np.array([0.0, 1.0, 0.999999, 0.25, 0.5, 1.0, 0.75, 1e-12])
+    # Table-bin edges: positions g + 0.5 + m/5 in cells (before rounding of
+    # uu/dx), so 5*|u_dist + KL| lands on or next to an integer.
+    g = rng.integers(kl + 1, 40, 40)
+    m = rng.integers(-2, 3, 40)
+    pos_u = g + 0.5 + m / 5.0
+    pos_v = rng.integers(kl + 1, 40, 40) + 0.5 + rng.integers(-2, 3, 40) / 5.0
+    bin_u = pos_u / NX
+    bin_v = pos_v / NX
+    bin_w = rng.uniform(0, 1, 40)
+    uu = np.concatenate([uu, edge_u, bin_u])
+    vv = np.concatenate([vv, edge_v, bin_v])
+    ww = np.concatenate([ww, edge_w, bin_w])
+    n = uu.size
+    vr = rng.normal(size=n * F * P).astype(np.float32)
+    vi = rng.normal(size=n * F * P).astype(np.float32)
+    vr[rng.choice(n * F * P, 25, replace=False)] = np.nan
+    wt = rng.uniform(0.25, 2.0, n * P).astype(np.float32)
+    return uu, vv, ww, vr, vi, wt
+
+
+def make_wstack():
+    for kind, name in ((O.GAUSS, "gauss"), (O.KAISERBESSEL, "kb"),
+                       (O.GAUSS_HI_PRECISION, "gausshp")):
+        for support in (7, 9):
+            uu, vv, ww, vr, vi, wt = wstack_inputs(100 + 10 * kind + support, support)
+            grid = O.ref_wstack(kind, PLANES, uu, vv, ww, vr, vi, wt, F, P, 1.0 / NX,
+                                1.0 / PLANES, support, NX, NY, threads=1)
+            np.savez_compressed(HERE / f"wstack_{name}_s{support}.npz", kind=kind,
+                                support=support, nx=NX, ny=NY, planes=PLANES, F=F, P=P, uu=uu,
+                                vv=vv, ww=ww, vis_real=vr, vis_img=vi, weight=wt, grid=grid)
+
+
+def make_phase():
+    nx, ny = 32, 24
+    for planes in (1, 4):
+        rng = np.random.default_rng(7 + planes)
+        gridss = rng.normal(size=2 * planes * nx * ny)
+        resolution, wmin, wmax = 1.0 / 700.0, -120.0, 340.0
+        for on in (True, False):
+            re, im = O.ref_phase_correction(gridss, nx, ny, planes, resolution, wmin, wmax,
+                                            phase_on=on, threads=1)
+            tag = "on" if on else "off"
+            np.savez_compressed(HERE / f"phase_{tag}_p{planes}.npz", nx=nx, ny=ny,
+                                planes=planes, resolution=resolution, wmin=wmin, wmax=wmax,
+                                phase_on=on, gridss=gridss, image_real=re, image_imag=im)
+
+
+TINY = synth.ObsConfig("tiny", "disk", 16, 1500.0, 45.0, 48, -3.0, 3.0, 45.0, 150e6, 152e6,
+                       2, 1, 3, 64, 4, synth.KAISERBESSEL, 7, seed=5)
+
+
+def make_chain():
+    ds = synth.make_dataset(TINY)
+    cfg = ds.cfg
+    grid = O.ref_wstack(cfg.kernel, cfg.planes, ds.uu, ds.vv, ds.ww, ds.vis_real, ds.vis_img,
+                        ds.weight, cfg.n_chan, cfg.n_pol, ds.dx, ds.dw, cfg.support, cfg.grid,
+                        cfg.grid, threads=1)
+    gridss = O.fft_planes(cfg.planes, cfg.grid, cfg.grid, grid, threads=1)
+    re, im = O.ref_phase_correction(gridss, cfg.grid, cfg.grid, cfg.planes, ds.resolution,
+                                    ds.wmin, ds.wmax, phase_on=True, threads=1)
+    np.savez_compressed(HERE / "chain_tiny.npz", kind=cfg.kernel, support=cfg.support,
+                        nx=cfg.grid, ny=cfg.grid, planes=cfg.planes, F=cfg.n_chan, P=cfg.n_pol,
+                        uu=ds.uu, vv=ds.vv, ww=ds.ww, vis_real=ds.vis_real, vis_img=ds.vis_img,
+                        weight=ds.weight, resolution=ds.resolution, wmin=ds.wmin, wmax=ds.wmax,
+                        grid=grid, gridss=gridss, image_real=re, image_imag=im)
+
+
+if __name__ == "__main__":
+    if not O.ref_available():
+        raise SystemExit("oracle/_ref is not built (needs the reference sources): "
+                         "make -C oracle ref")
+    make_wstack()
+    make_phase()
+    make_chain()
+    for f in sorted(HERE.glob("*.npz")):
+        print(f.name, f.stat().st_size)

@@ -1,0 +1,86 @@
+"""librick_hip.so: it loads, exports every entry point include/rick.h declares,
+and rejects bad arguments before touching a device (no GPU needed)."""
+import ctypes
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+from rick_amd import _lib
+
+HEADER = ROOT / "include" / "rick.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?(?:void|int|size_t|char)\s*\*?\s*(\w+)\s*\(", text,
+                       flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_is_built():
+    assert _lib.LIB_PATH.exists(), "run `python -m rick_amd.build`"
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared_functions()
+    assert {"wstack", "phase_correction", "rick_fft_planes", "rick_plan_create",
+            "rick_plan_grid", "rick_plan_fft", "rick_plan_phase", "rick_plan_image"} <= set(names)
+    out = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    # The ctypes binding declares every one of them.
+    assert set(names) == set(_lib.SIGNATURES)
+    L = _lib.lib()
+    for n in names:
+        assert hasattr(L, n)
+
+
+def test_no_oracle_dependency():
+    out = subprocess.run(["ldd", str(_lib.LIB_PATH)], capture_output=True, text=True).stdout
+    assert "oracle" not in out and "libref" not in out
+
+
+def test_argument_errors_without_device():
+    L = _lib.lib()
+    assert L.rick_set_kernel(7) == _lib.RICK_ERR_INVALID
+    assert b"unknown kernel" in L.rick_last_error()
+    assert L.rick_set_kernel(_lib.KAISERBESSEL) == _lib.RICK_OK
+    assert L.rick_get_kernel() == _lib.KAISERBESSEL
+    assert L.rick_version().startswith(b"rick")
+
+    def create(**kw):
+        d = _lib.PlanDesc(num_w_planes=4, grid_size_x=64, grid_size_y=64, kernel=1,
+                          w_support=7, dx=1 / 64, dw=1 / 4, precision=0, max_points=100)
+        for k, v in kw.items():
+            setattr(d, k, v)
+        h = ctypes.c_void_p()
+        return L.rick_plan_create(ctypes.byref(h), ctypes.byref(d))
+
+    assert create(w_support=8) == _lib.RICK_ERR_INVALID          # even support
+    assert create(w_support=17) == _lib.RICK_ERR_INVALID
+    assert create(kernel=5) == _lib.RICK_ERR_INVALID
+    assert create(num_w_planes=0) == _lib.RICK_ERR_INVALID
+    assert create(precision=3) == _lib.RICK_ERR_INVALID
+    assert create(dx=0.0) == _lib.RICK_ERR_INVALID
+    assert create(tile_u=48, tile_v=48) == _lib.RICK_ERR_INVALID
+    assert create(max_points=2 ** 31) == _lib.RICK_ERR_INVALID
+    assert L.rick_plan_grid(None, 0, 1, 1, *([None] * 7), 0, None) == _lib.RICK_ERR_INVALID
+
+
+def test_python_wrappers_validate_arrays():
+    import numpy as np
+    from rick_amd import imaging
+    g = np.zeros(2 * 4 * 8 * 8)
+    with pytest.raises(TypeError):
+        imaging.wstack(4, 1, 1, 1, np.zeros(1, np.float32), np.zeros(1), np.zeros(1),
+                       np.zeros(1, np.float32), np.zeros(1, np.float32), np.ones(1, np.float32),
+                       1 / 8, 1 / 4, 7, 8, 8, g)
+    with pytest.raises(ValueError):
+        imaging.wstack(4, 3, 1, 1, np.zeros(1), np.zeros(1), np.zeros(1),
+                       np.zeros(1, np.float32), np.zeros(1, np.float32), np.ones(1, np.float32),
+                       1 / 8, 1 / 4, 7, 8, 8, g)

@@ -1,0 +1,223 @@
+"""Parity of the HIP path with the reference's outputs and the CPU oracle.
+
+Bars: FP64 grid / image relative L2 <= 1e-12 against the reference-produced
+vectors and the oracle (only the atomic/reduction order differs); FP32 path
+relative L2 <= 1e-5 against the FP64 result.  FFT: against numpy's ifft2
+(pocketfft) since FFTW, the reference's FFT, is not available.
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import load_golden, rel_l2
+from oracle import oracle as O
+import rick_amd
+from rick_amd import FP32, FP64, Plan
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 1e-5
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def grid_plan(z, precision=FP64, **kw):
+    return Plan(int(z["nx"]), int(z["ny"]), int(z["planes"]), kernel=int(z["kind"]),
+                support=int(z["support"]), dx=1.0 / int(z["nx"]), dw=1.0 / int(z["planes"]),
+                precision=precision, max_points=int(z["uu"].size), **kw)
+
+
+def grid_inputs(z):
+    return [dev(z[k]) for k in ("uu", "vv", "ww", "vis_real", "vis_img", "weight")]
+
+
+def nan_grid(plan):
+    g = plan.new_grid()
+    g.fill_(float("nan"))
+    return g
+
+
+# --- gridding -----------------------------------------------------------------
+
+@pytest.mark.parametrize("kernel", ["gauss", "kb", "gausshp"])
+@pytest.mark.parametrize("support", [7, 9])
+def test_grid_fp64_matches_reference_vectors(kernel, support):
+    z = load_golden(f"wstack_{kernel}_s{support}.npz")
+    print(f"case wstack_{kernel}_s{support}", flush=True)
+    plan = grid_plan(z)
+    g = plan.grid(*grid_inputs(z), int(z["F"]), int(z["P"]), out=nan_grid(plan))
+    got = host(g)
+    assert np.isfinite(got).all()
+    assert rel_l2(got, z["grid"]) <= FP64_TOL
+    assert np.abs(got - z["grid"]).max() <= 1e-13 * np.abs(z["grid"]).max()
+    assert np.array_equal(got == 0, z["grid"] == 0)
+
+
+@pytest.mark.parametrize("tile", [(64, 64), (64, 32), (32, 32), (128, 32), (32, 64)])
+@pytest.mark.parametrize("rows_per_task", [0, 5])
+def test_grid_tiles_and_split_buckets(tile, rows_per_task):
+    # rows_per_task=5 splits almost every bucket: exercises the atomic flush.
+    z = load_golden("wstack_kb_s7.npz")
+    print(f"case tile={tile} rows_per_task={rows_per_task}", flush=True)
+    plan = grid_plan(z, tile=tile, rows_per_task=rows_per_task)
+    got = host(plan.grid(*grid_inputs(z), int(z["F"]), int(z["P"]), out=nan_grid(plan)))
+    assert rel_l2(got, z["grid"]) <= FP64_TOL
+
+
+def test_grid_accumulate_and_empty_input():
+    z = load_golden("wstack_gauss_s7.npz")
+    plan = grid_plan(z)
+    ins = grid_inputs(z)
+    g = plan.grid(*ins, int(z["F"]), int(z["P"]))
+    plan.grid(*ins, int(z["F"]), int(z["P"]), out=g, accumulate=True)
+    assert rel_l2(host(g), 2 * z["grid"]) <= FP64_TOL
+    empty = [t[:0] for t in ins]
+    g = plan.grid(*empty, int(z["F"]), int(z["P"]), out=nan_grid(plan))
+    assert not host(g).any()
+
+
+def test_grid_fp32_within_tolerance():
+    z = load_golden("wstack_kb_s9.npz")
+    plan = grid_plan(z, precision=FP32)
+    got = host(plan.grid(*grid_inputs(z), int(z["F"]), int(z["P"]), out=nan_grid(plan)))
+    assert got.dtype == np.float32
+    assert rel_l2(got, z["grid"]) <= 1e-6
+
+
+def test_dropin_wstack_accumulates_into_host_grid():
+    z = load_golden("wstack_gausshp_s9.npz")
+    rick_amd.set_kernel(int(z["kind"]))
+    try:
+        pre = np.random.default_rng(0).normal(size=z["grid"].size)
+        g = pre.copy()
+        rick_amd.wstack(int(z["planes"]), int(z["uu"].size), int(z["F"]), int(z["P"]), z["uu"],
+                        z["vv"], z["ww"], z["vis_real"], z["vis_img"], z["weight"],
+                        1.0 / int(z["nx"]), 1.0 / int(z["planes"]), int(z["support"]),
+                        int(z["nx"]), int(z["ny"]), g, 4, 0)
+        assert rel_l2(g - pre, z["grid"]) <= 1e-12
+    finally:
+        rick_amd.set_kernel(rick_amd.KAISERBESSEL)
+
+
+# --- FFT ------------------------------------------------------------------------
+
+def cplx(g, planes, ny, nx):
+    x = g.reshape(planes, ny, nx, 2).astype(np.float64)
+    return x[..., 0] + 1j * x[..., 1]
+
+
+@pytest.mark.parametrize("shape", [(16, 16), (32, 64), (64, 32), (512, 512), (1024, 256),
+                                   (2048, 64), (64, 4096), (8192, 16), (16, 8192)])
+@pytest.mark.parametrize("precision", [FP64, FP32])
+@pytest.mark.parametrize("inplace", [False, True])
+def test_fft_matches_numpy(shape, precision, inplace):
+    ny, nx = shape
+    planes = 3
+    print(f"case fft {shape} prec={precision} inplace={inplace}", flush=True)
+    rng = np.random.default_rng(ny + nx)
+    c = rng.normal(size=(planes, ny, nx)) + 1j * rng.normal(size=(planes, ny, nx))
+    flat = np.stack([c.real, c.imag], -1).reshape(-1)
+    plan = Plan(nx, ny, planes, precision=precision)
+    src = dev(flat if precision == FP64 else flat.astype(np.float32))
+    out = src if inplace else torch.empty_like(src)
+    plan.fft(src, out)
+    got = cplx(host(out), planes, ny, nx)
+    ref = np.fft.ifft2(c, axes=(1, 2))
+    assert rel_l2(got, ref) <= (1e-14 if precision == FP64 else 2e-6)
+
+
+def test_dropin_fft_planes():
+    z = load_golden("chain_tiny.npz")
+    gs = rick_amd.fft_planes(int(z["planes"]), int(z["nx"]), int(z["ny"]), z["grid"])
+    assert rel_l2(gs, z["gridss"]) <= 1e-14
+
+
+# --- phase correction ---------------------------------------------------------
+
+@pytest.mark.parametrize("tag", ["on_p1", "on_p4", "off_p1", "off_p4"])
+def test_phase_matches_reference_vectors(tag):
+    z = load_golden(f"phase_{tag}.npz")
+    nx, ny, planes = int(z["nx"]), int(z["ny"]), int(z["planes"])
+    plan = Plan(nx, ny, planes, resolution=float(z["resolution"]), wmin=float(z["wmin"]),
+                wmax=float(z["wmax"]), phase_on=bool(z["phase_on"]))
+    re, im = plan.phase(dev(z["gridss"]))
+    assert rel_l2(host(re) + 1j * host(im), z["image_real"] + 1j * z["image_imag"]) <= 1e-14
+    # FP32 plane stack: phase argument still FP64.
+    plan32 = Plan(nx, ny, planes, resolution=float(z["resolution"]), wmin=float(z["wmin"]),
+                  wmax=float(z["wmax"]), phase_on=bool(z["phase_on"]), precision=FP32)
+    re, im = plan32.phase(dev(z["gridss"].astype(np.float32)))
+    assert rel_l2(host(re) + 1j * host(im), z["image_real"] + 1j * z["image_imag"]) <= 1e-6
+
+
+def test_dropin_phase_correction_accumulates():
+    z = load_golden("phase_on_p4.npz")
+    nx, ny, planes = int(z["nx"]), int(z["ny"]), int(z["planes"])
+    re = np.ones(nx * ny)
+    im = np.full(nx * ny, -2.0)
+    rick_amd.set_phase(True)
+    rick_amd.phase_correction(z["gridss"], re, im, nx, ny, planes, nx, ny,
+                              float(z["resolution"]), float(z["wmin"]), float(z["wmax"]), 2, 0)
+    assert rel_l2(re - 1, z["image_real"]) <= 1e-14
+    assert rel_l2(im + 2, z["image_imag"]) <= 1e-14
+
+
+def test_phase_rejects_degenerate_w_range():
+    plan = Plan(16, 16, 4, wmin=5.0, wmax=5.0)
+    with pytest.raises(rick_amd.RickError):
+        plan.phase(torch.zeros(2 * 4 * 16 * 16, dtype=torch.float64, device="cuda"))
+
+
+# --- whole chain --------------------------------------------------------------
+
+def chain(z, precision):
+    nx, ny, planes = int(z["nx"]), int(z["ny"]), int(z["planes"])
+    plan = Plan(nx, ny, planes, kernel=int(z["kind"]), support=int(z["support"]), dx=1.0 / nx,
+                dw=1.0 / planes, precision=precision, max_points=int(z["uu"].size),
+                resolution=float(z["resolution"]), wmin=float(z["wmin"]), wmax=float(z["wmax"]))
+    re, im, _ = plan.image(*grid_inputs(z), int(z["F"]), int(z["P"]))
+    return host(re) + 1j * host(im)
+
+
+def test_chain_tiny_matches_reference_vectors():
+    z = load_golden("chain_tiny.npz")
+    ref = z["image_real"] + 1j * z["image_imag"]
+    assert rel_l2(chain(z, FP64), ref) <= FP64_TOL
+    assert rel_l2(chain(z, FP32), ref) <= FP32_TOL
+
+
+@pytest.fixture(scope="module")
+def small_case():
+    from rick_amd import synth
+    ds = synth.make_dataset(synth.CONFIGS["small"])
+    cfg = ds.cfg
+    grid = O.wstack(cfg.kernel, cfg.planes, ds.uu, ds.vv, ds.ww, ds.vis_real, ds.vis_img,
+                    ds.weight, cfg.n_chan, cfg.n_pol, ds.dx, ds.dw, cfg.support, cfg.grid,
+                    cfg.grid, threads=8)
+    gridss = O.fft_planes(cfg.planes, cfg.grid, cfg.grid, grid, threads=8)
+    re, im = O.phase_correction(gridss, cfg.grid, cfg.grid, cfg.planes, ds.resolution, ds.wmin,
+                                ds.wmax, threads=8)
+    return ds, grid, re + 1j * im
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32])
+def test_small_config_matches_oracle(small_case, precision):
+    """512^2 x 8 planes, 516,096 visibilities, Gaussian support 7."""
+    ds, grid_ref, img_ref = small_case
+    print(f"case small precision={precision}", flush=True)
+    plan = rick_amd.plan_for(ds, precision=precision)
+    ins = [dev(a) for a in (ds.uu, ds.vv, ds.ww, ds.vis_real, ds.vis_img, ds.weight)]
+    g = plan.grid(*ins, ds.cfg.n_chan, ds.cfg.n_pol)
+    e_grid = rel_l2(host(g), grid_ref)
+    re, im, _ = plan.image(*ins, ds.cfg.n_chan, ds.cfg.n_pol)
+    e_img = rel_l2(host(re) + 1j * host(im), img_ref)
+    print(f"small precision={precision}: grid rel-L2 {e_grid:.3e}, image rel-L2 {e_img:.3e}")
+    tol = FP64_TOL if precision == FP64 else FP32_TOL
+    assert e_grid <= tol and e_img <= tol
