@@ -1,0 +1,250 @@
+"""Benchmark of the imaging chain on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config lofar]
+
+Workload (default): the LOFAR-like observation -- 62 stations (dense 2 km core,
+log-radial to 50 km), 1,891 baselines x 4,096 times x 16 channels
+(120-136 MHz) x 1 polarization = 123,928,576 visibilities, Kaiser-Bessel
+support 7, a 4096 x 4096 grid with 32 w-planes -- generated synthetically on
+the GPU (seeded), inputs resident in HBM before timing.
+
+A step is one gridding pass (wstack) over all visibilities.  It is timed for
+the FP64 path (the reference's arithmetic) and the FP32 path; ``value`` is the
+FP32 path's gridded visibilities per second.  The whole image chain
+(grid -> per-plane FFT -> phase correction) is timed beside it, and the FP32
+image is compared with the FP64 one.  Per-kernel durations come from HIP
+events on the launch stream (Plan.profile); the roofline object describes the
+dominant gridding kernel.  The CPU baseline is the reference's own wstack()
+(compiled from its sources into oracle/_ref) with OpenMP on this host's cores
+over a bounded sample of the same rows.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="lofar")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="target CPU time of the bounded CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["fp64", "fp32"], default=None,
+                    help="run one path only (for profiler runs)")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def timed(fn, steps, torch):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def run_path(ds, ins, precision, steps, warmup, torch):
+    from rick_amd import imaging
+    cfg = ds.cfg
+    F, P = cfg.n_chan, cfg.n_pol
+    plan = imaging.plan_for(ds, precision=precision)
+    grid = plan.new_grid()
+    npix = cfg.grid * cfg.grid
+    re = torch.empty(npix, dtype=torch.float64, device="cuda:0")
+    im = torch.empty(npix, dtype=torch.float64, device="cuda:0")
+
+    def grid_step():
+        plan.grid(*ins, F, P, out=grid)
+
+    def image_step():
+        plan.image(*ins, F, P, grid=grid, image_real=re, image_imag=im)
+
+    for _ in range(warmup):
+        image_step()
+    grid_ms = timed(grid_step, steps, torch)
+    image_ms = timed(image_step, steps, torch)
+    # Per-kernel durations (HIP events on the launch stream).
+    plan.profile(True)
+    for _ in range(steps):
+        image_step()
+    st = plan.stage_times()
+    plan.profile(False)
+    stats = plan.grid_stats()
+    stages = {k: {"ms_per_launch": (ms / n if n else 0.0), "launches_per_step": n / steps,
+                  "ms_per_step": ms / steps}
+              for k, (ms, n) in st.items() if n}
+    img = (re.cpu().numpy() + 1j * im.cpu().numpy())
+    return plan, grid_ms, image_ms, stages, stats, img
+
+
+def cpu_baseline(ds_host_fn, cfg, target_s):
+    """Reference wstack() (oracle/_ref) on a bounded sample of the same rows."""
+    from oracle import oracle as O
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    kind = "reference" if O.ref_available() else "port"
+    nx = cfg.grid
+    grid = np.zeros(2 * nx * nx * (cfg.planes + 1), dtype=np.float64)
+
+    def run(n):
+        uu, vv, ww, vr, vi, wt = ds_host_fn(n)
+        t0 = time.perf_counter()
+        if kind == "reference":
+            O.ref_wstack_into(cfg.kernel, cfg.planes, uu, vv, ww, vr, vi, wt, cfg.n_chan,
+                              cfg.n_pol, 1.0 / nx, 1.0 / cfg.planes, cfg.support, nx, nx, grid,
+                              threads)
+        else:
+            O.wstack(cfg.kernel, cfg.planes, uu, vv, ww, vr, vi, wt, cfg.n_chan, cfg.n_pol,
+                     1.0 / nx, 1.0 / cfg.planes, cfg.support, nx, nx, grid=grid, threads=threads)
+        return time.perf_counter() - t0
+
+    n = 16384
+    t = run(n)
+    while t < 1.0 and n < cfg.n_rows:
+        n = min(cfg.n_rows, n * 4)
+        t = run(n)
+    n = int(min(cfg.n_rows, max(n, n * target_s / max(t, 1e-6))))
+    t = run(n)
+    vis = n * cfg.n_chan * cfg.n_pol
+    return {"value": vis / t, "unit": "vis/s", "cores": threads, "kind": kind,
+            "sample": f"{n} of {cfg.n_rows} rows ({vis} vis, time-major block from the middle of "
+                      f"the observation) gridded by the reference's wstack() "
+                      f"(KAISERBESSEL, support {cfg.support}) into the full {nx}^2 x "
+                      f"{cfg.planes}-plane FP64 grid with {threads} OpenMP threads; "
+                      f"{t:.2f} s", "cpu": _cpu_model()}
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark measures one GPU; --gpus must be 1 (the "
+                          "multi-GPU design shards w-planes and is not built)"}))
+        sys.exit(2)
+    import torch
+
+    from rick_amd import FP32, FP64, synth
+
+    cfg = synth.CONFIGS[args.config]
+    log(f"generating {cfg.name}: {cfg.n_rows} rows x {cfg.n_chan} ch = {cfg.n_vis} vis")
+    ds = synth.make_dataset(cfg, device="cuda:0")
+    ins = [ds.uu, ds.vv, ds.ww, ds.vis_real, ds.vis_img, ds.weight]
+    torch.cuda.synchronize()
+    vis = cfg.n_vis
+    res = {}
+    for name, prec in (("fp64", FP64), ("fp32", FP32)):
+        if args.only and args.only != name:
+            continue
+        log(f"{name}: warmup {args.warmup}, steps {args.steps}")
+        plan, gms, ims, stages, stats, img = run_path(ds, ins, prec, args.steps, args.warmup,
+                                                      torch)
+        res[name] = {"grid_ms": gms, "image_ms": ims, "grid_vis_per_s": vis / (gms * 1e-3),
+                     "stages": stages, "grid_stats": stats, "img": img,
+                     "grid_bytes": plan.grid_bytes}
+        log(f"{name}: grid {gms:.3f} ms ({vis / gms / 1e6:.3f} Gvis/s), image {ims:.3f} ms")
+        del plan
+        torch.cuda.empty_cache()
+
+    main_path = "fp32" if "fp32" in res else "fp64"
+    r = res[main_path]
+    rows, F, P = cfg.n_rows, cfg.n_chan, cfg.n_pol
+    cb = 8 if main_path == "fp32" else 16
+    # Dominant gridding kernel and its algorithmic bytes.
+    grid_stages = {k: v for k, v in r["stages"].items()
+                   if k in ("count", "scan", "scatter", "zero_split", "tile")}
+    dom = max(grid_stages, key=lambda k: grid_stages[k]["ms_per_step"])
+    grid_bytes = cfg.planes * cfg.grid * cfg.grid * cb
+    alg = {"tile": grid_bytes,                              # every grid cell written once
+           "scatter": rows * (24 + 4 * P) + vis * 8,        # uvw, weights, visibilities read once
+           "count": rows * 24}.get(dom)
+    t_dom = grid_stages[dom]["ms_per_launch"] * 1e-3
+    achieved = alg / t_dom / 1e9 if alg else None
+    stage_alg = rows * (24 + 4 * P) + vis * 8 + grid_bytes
+    out = {
+        "metric": "gridded vis/s + image time, FP32 vs FP64 path at rel-L2<=1e-5; serve p99",
+        "value": r["grid_vis_per_s"],
+        "unit": "vis/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["grid_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if main_path == "fp32" else "f64",
+        "data": "synthetic (seeded Earth-rotation uvw, 20 point sources + noise), generated on "
+                "the GPU",
+        "config": {"workload": f"{cfg.name}-like: {cfg.n_ant} stations, {cfg.n_baselines} "
+                               f"baselines x {cfg.n_times} times x {cfg.n_chan} channels = "
+                               f"{vis} visibilities; {cfg.grid}^2 x {cfg.planes} w-planes, "
+                               f"Kaiser-Bessel support {cfg.support}",
+                   "step": "one wstack() gridding pass over all visibilities",
+                   "path": main_path},
+        "roofline": {"bound": "hbm", "kernel": dom,
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": None,
+                     "alg_bytes_per_launch": alg,
+                     "ms_per_launch": grid_stages[dom]["ms_per_launch"]},
+        "gridding_stage": {"alg_bytes": stage_alg,
+                           "achieved_GBs": stage_alg / (r["grid_ms"] * 1e-3) / 1e9,
+                           "frac_of_peak": stage_alg / (r["grid_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+    }
+    for name, v in res.items():
+        out[name] = {"grid_ms": v["grid_ms"], "image_ms": v["image_ms"],
+                     "grid_vis_per_s": v["grid_vis_per_s"], "grid_stats": v["grid_stats"],
+                     "stages_ms_per_step": {k: round(s["ms_per_step"], 4)
+                                            for k, s in v["stages"].items()}}
+    if "fp32" in res and "fp64" in res:
+        a, b = res["fp32"]["img"], res["fp64"]["img"]
+        out["fp32_vs_fp64"] = {
+            "grid_speedup": res["fp64"]["grid_ms"] / res["fp32"]["grid_ms"],
+            "image_speedup": res["fp64"]["image_ms"] / res["fp32"]["image_ms"],
+            "image_rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+            "image_peak_rel": float(np.abs(a - b).max() / np.abs(b).max()),
+        }
+    if not args.no_cpu:
+        log("cpu baseline")
+        mid = cfg.n_rows // 2
+
+        def host_rows(n):
+            s = max(0, min(mid, cfg.n_rows - n))
+            sub = ds.subset(slice(s, s + n)).to_numpy()
+            return (np.ascontiguousarray(sub.uu), np.ascontiguousarray(sub.vv),
+                    np.ascontiguousarray(sub.ww), np.ascontiguousarray(sub.vis_real),
+                    np.ascontiguousarray(sub.vis_img), np.ascontiguousarray(sub.weight))
+
+        out["cpu_baseline"] = cpu_baseline(host_rows, cfg, args.cpu_seconds)
+        out["gpu_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -143,6 +143,31 @@ int rick_plan_fft(rick_plan plan, const void *in, void *out, void *stream);
 int rick_plan_phase(rick_plan plan, const void *gridss, double *image_real,
                     double *image_imag, int accumulate, void *stream);
 
+/* Per-stage kernel timing.  While enabled, every kernel the plan launches is
+ * bracketed by HIP events on its stream; rick_plan_stage_times() waits for the
+ * recorded work and returns, per stage, the summed kernel time (ms) and the
+ * number of launches since the previous query, then resets. */
+enum {
+    RICK_STAGE_COUNT_ROWS = 0, /* bucket histogram (incl. clearing it)        */
+    RICK_STAGE_SCAN = 1,       /* bucket / task offsets                        */
+    RICK_STAGE_SCATTER = 2,    /* per-row channel sums -> sorted records       */
+    RICK_STAGE_ZERO = 3,       /* clear tiles of split buckets                 */
+    RICK_STAGE_TILE = 4,       /* LDS-tile accumulation and grid write         */
+    RICK_STAGE_FFT_COLS = 5,   /* column FFT (single pass, or four-step A)     */
+    RICK_STAGE_FFT_COLS_B = 6, /* column FFT four-step B                       */
+    RICK_STAGE_FFT_ROWS = 7,   /* row FFT (+ 1/(Nx*Ny))                         */
+    RICK_STAGE_PHASE = 8,      /* phase correction + plane sum                 */
+    RICK_STAGE_COUNT = 9
+};
+int rick_plan_profile(rick_plan plan, int enable);
+int rick_plan_stage_times(rick_plan plan, double *ms, int *launches, int n);
+
+/* Binning statistics of the last rick_plan_grid call (waits for it):
+ * stats[0] = row records written (rows x tiles each row touches),
+ * stats[1] = gridding tasks (workgroups), stats[2] = buckets split over
+ * several tasks.  n <= 3 entries are written. */
+int rick_plan_grid_stats(rick_plan plan, unsigned int *stats, int n);
+
 /* Whole chain: grid (overwrite) -> FFT in place -> phase into the images
  * (overwrite).  grid is caller scratch of rick_plan_grid_bytes(). */
 int rick_plan_image(rick_plan plan, unsigned int num_points, unsigned int freq_per_chan,

@@ -184,6 +184,18 @@ def ref_wstack(kind, planes, uu, vv, ww, vis_real, vis_img, weight, freq_per_cha
     return grid[: plane * planes].copy()
 
 
+def ref_wstack_into(kind, planes, uu, vv, ww, vis_real, vis_img, weight, freq_per_chan,
+                    polarizations, dx, dw, support, nx, ny, grid, threads):
+    """The reference's wstack() accumulating into ``grid``, which must hold
+    planes+1 planes (the extra one absorbs rows at w == 1.0, see ref_wstack)."""
+    assert grid.dtype == np.float64 and grid.size >= 2 * nx * ny * (planes + 1)
+    _ref(KERNEL_NAMES[kind]).wstack(planes, uu.shape[0], freq_per_chan, polarizations,
+                                    _ptr(uu, _dp), _ptr(vv, _dp), _ptr(ww, _dp),
+                                    _ptr(vis_real, _fp), _ptr(vis_img, _fp), _ptr(weight, _fp),
+                                    dx, dw, support, nx, ny, _ptr(grid, _dp), threads, 0)
+    return grid
+
+
 def ref_phase_correction(gridss, nx, ny, planes, resolution, wmin, wmax, phase_on=True,
                          threads=1):
     gridss = _c(gridss, np.float64)

@@ -13,6 +13,8 @@ LIB_PATH = Path(__file__).resolve().parent / "librick_hip.so"
 RICK_OK, RICK_ERR_INVALID, RICK_ERR_UNSUPPORTED, RICK_ERR_DEVICE, RICK_ERR_NOMEM = range(5)
 GAUSS, KAISERBESSEL, GAUSS_HI_PRECISION = 0, 1, 2
 FP64, FP32 = 0, 1
+STAGES = ("count", "scan", "scatter", "zero_split", "tile", "fft_cols", "fft_cols_b",
+          "fft_rows", "phase")
 
 _c_int, _c_uint, _c_double, _c_size, _vp = (ctypes.c_int, ctypes.c_uint, ctypes.c_double,
                                             ctypes.c_size_t, ctypes.c_void_p)
@@ -54,6 +56,9 @@ SIGNATURES = {
                                 _vp, _c_int, _vp]),
     "rick_plan_fft": (_c_int, [_vp, _vp, _vp, _vp]),
     "rick_plan_phase": (_c_int, [_vp, _vp, _vp, _vp, _c_int, _vp]),
+    "rick_plan_profile": (_c_int, [_vp, _c_int]),
+    "rick_plan_stage_times": (_c_int, [_vp, _dp, ctypes.POINTER(_c_int), _c_int]),
+    "rick_plan_grid_stats": (_c_int, [_vp, ctypes.POINTER(_c_uint), _c_int]),
     "rick_plan_image": (_c_int, [_vp, _c_uint, _c_uint, _c_uint, _vp, _vp, _vp, _vp, _vp, _vp,
                                  _vp, _vp, _vp, _vp]),
 }

@@ -72,6 +72,7 @@ struct rick_plan_s {
     void *tmp_plane = nullptr;
     bool fft_ok = false;
     bool grid_ok = false;
+    Timing timing;
 };
 
 extern "C" {
@@ -111,6 +112,11 @@ int rick_plan_destroy(rick_plan p)
     hipGetDevice(&dev);
     hipSetDevice(p->d.device);
     if (p->dmem) hipFree(p->dmem);
+    for (auto &v : p->timing.ev)
+        for (auto &e : v) {
+            hipEventDestroy(e.first);
+            hipEventDestroy(e.second);
+        }
     hipSetDevice(dev);
     delete p;
     return RICK_OK;
@@ -267,7 +273,8 @@ int rick_plan_grid(rick_plan p, unsigned int n, unsigned int F, unsigned int P, 
         return set_error(RICK_ERR_INVALID, "null input array");
     RICK_HIP(hipSetDevice(p->d.device));
     RICK_HIP(launch_grid(p->geom, p->work, p->d.precision, p->table_d, n, F, P, uu, vv, ww, vr,
-                         vi, wt, grid, accumulate, static_cast<hipStream_t>(stream)));
+                         vi, wt, grid, accumulate, static_cast<hipStream_t>(stream),
+                         &p->timing));
     return RICK_OK;
 }
 
@@ -284,7 +291,7 @@ int rick_plan_fft(rick_plan p, const void *in, void *out, void *stream)
     // fourier_transform.c:24
     const double norm = 1.0 / (double)((double)p->geom.nx * (double)p->geom.ny);
     RICK_HIP(launch_fft(p->fft, p->d.precision, in, out, p->tmp_plane, norm,
-                        static_cast<hipStream_t>(stream)));
+                        static_cast<hipStream_t>(stream), &p->timing));
     return RICK_OK;
 }
 
@@ -298,7 +305,52 @@ int rick_plan_phase(rick_plan p, const void *gridss, double *re, double *im, int
         return set_error(RICK_ERR_INVALID, "wmax == wmin: the w-plane spacing is undefined");
     RICK_HIP(hipSetDevice(p->d.device));
     RICK_HIP(launch_phase(p->phase, p->d.precision, gridss, re, im, accumulate,
-                          static_cast<hipStream_t>(stream)));
+                          static_cast<hipStream_t>(stream), &p->timing));
+    return RICK_OK;
+}
+
+int rick_plan_profile(rick_plan p, int enable)
+{
+    clear_error();
+    if (!p) return set_error(RICK_ERR_INVALID, "null plan");
+    p->timing.on = enable != 0;
+    for (auto &u : p->timing.used) u = 0;
+    return RICK_OK;
+}
+
+int rick_plan_stage_times(rick_plan p, double *ms, int *launches, int n)
+{
+    clear_error();
+    if (!p) return set_error(RICK_ERR_INVALID, "null plan");
+    RICK_HIP(hipSetDevice(p->d.device));
+    for (int st = 0; st < RICK_STAGE_COUNT; ++st) {
+        double sum = 0.0;
+        for (size_t k = 0; k < p->timing.used[st]; ++k) {
+            const auto &e = p->timing.ev[st][k];
+            RICK_HIP(hipEventSynchronize(e.second));
+            float t = 0.0f;
+            RICK_HIP(hipEventElapsedTime(&t, e.first, e.second));
+            sum += t;
+        }
+        if (st < n) {
+            if (ms) ms[st] = sum;
+            if (launches) launches[st] = (int)p->timing.used[st];
+        }
+        p->timing.used[st] = 0;
+    }
+    return RICK_OK;
+}
+
+int rick_plan_grid_stats(rick_plan p, unsigned int *stats, int n)
+{
+    clear_error();
+    if (!p || !stats) return set_error(RICK_ERR_INVALID, "null argument");
+    if (!p->grid_ok) return set_error(RICK_ERR_INVALID, "plan has no gridding capacity");
+    RICK_HIP(hipSetDevice(p->d.device));
+    RICK_HIP(hipDeviceSynchronize());
+    uint32_t meta[4] = {0, 0, 0, 0};
+    RICK_HIP(hipMemcpy(meta, p->work.meta, sizeof meta, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n && i < 3; i++) stats[i] = meta[i];
     return RICK_OK;
 }
 

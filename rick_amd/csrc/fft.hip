@@ -332,7 +332,7 @@ hipError_t allow_lds(K kernel, size_t bytes)
 
 template <typename T>
 hipError_t fft_impl(const FftPlan &p, const void *in_v, void *out_v, void *tmp_v,
-                    double scale_d, hipStream_t s)
+                    double scale_d, hipStream_t s, Timing *tm)
 {
     using CT = C2<T>;
     const CT *in = static_cast<const CT *>(in_v);
@@ -364,8 +364,10 @@ hipError_t fft_impl(const FftPlan &p, const void *in_v, void *out_v, void *tmp_v
             const int threads = a.L * a.W / kE;
             const size_t lds = sizeof(CT) * (size_t)a.L * a.W;
             if (hipError_t e = allow_lds(k_cols<T>, lds); e != hipSuccess) return e;
+            tm->begin(RICK_STAGE_FFT_COLS, s);
             hipLaunchKernelGGL(k_cols<T>, dim3(nx / a.W, 1, planes), dim3(threads), lds, s,
                                plan_passes(a.L), a, twy, src, out, one);
+            tm->end(RICK_STAGE_FFT_COLS, s);
         } else {
             // Four-step: ny = N1 * N2 with N1 = 64.
             const int N1 = 64, N2 = ny / 64;
@@ -401,20 +403,28 @@ hipError_t fft_impl(const FftPlan &p, const void *in_v, void *out_v, void *tmp_v
                 return e;
             if (in_v != out_v) {
                 // A: in -> out (rows permuted), B: in place on out.
+                tm->begin(RICK_STAGE_FFT_COLS, s);
                 hipLaunchKernelGGL(k_cols<T>, dim3(nx / A.W, N2, planes), dim3(A.L * A.W / kE),
                                    ldsA, s, PA, A, twy, src, out, one);
+                tm->end(RICK_STAGE_FFT_COLS, s);
+                tm->begin(RICK_STAGE_FFT_COLS_B, s);
                 hipLaunchKernelGGL(k_cols<T>, dim3(nx / B.W, N1, planes), dim3(B.L * B.W / kE),
                                    ldsB, s, PB, B, twy, out, out, one);
+                tm->end(RICK_STAGE_FFT_COLS_B, s);
             } else {
                 // In place: per plane, A: plane -> tmp, B: tmp -> plane.
                 if (tmp_v == nullptr) return hipErrorInvalidValue;
                 CT *tmp = static_cast<CT *>(tmp_v);
                 const size_t pl = (size_t)nx * ny;
                 for (int iw = 0; iw < planes; ++iw) {
+                    tm->begin(RICK_STAGE_FFT_COLS, s);
                     hipLaunchKernelGGL(k_cols<T>, dim3(nx / A.W, N2, 1), dim3(A.L * A.W / kE),
                                        ldsA, s, PA, A, twy, out + iw * pl, tmp, one);
+                    tm->end(RICK_STAGE_FFT_COLS, s);
+                    tm->begin(RICK_STAGE_FFT_COLS_B, s);
                     hipLaunchKernelGGL(k_cols<T>, dim3(nx / B.W, N1, 1), dim3(B.L * B.W / kE),
                                        ldsB, s, PB, B, twy, tmp, out + iw * pl, one);
+                    tm->end(RICK_STAGE_FFT_COLS_B, s);
                 }
             }
         }
@@ -429,8 +439,10 @@ hipError_t fft_impl(const FftPlan &p, const void *in_v, void *out_v, void *tmp_v
         const int LP = lpad(nx) + 1;
         const size_t lds = sizeof(CT) * (size_t)LP * G;
         if (hipError_t e = allow_lds(k_rows<T>, lds); e != hipSuccess) return e;
+        tm->begin(RICK_STAGE_FFT_ROWS, s);
         hipLaunchKernelGGL(k_rows<T>, dim3(rows_total / G), dim3(threads), lds, s,
                            plan_passes(nx), nx, G, rows_total, 1, twx, src, out, scale);
+        tm->end(RICK_STAGE_FFT_ROWS, s);
     } else if (src != out || scale != one) {
         return hipErrorInvalidValue;
     }
@@ -446,10 +458,12 @@ bool fft_needs_tmp(int precision, int ny)
 }
 
 hipError_t launch_fft(const FftPlan &p, int precision, const void *in, void *out, void *tmp,
-                      double scale, hipStream_t s)
+                      double scale, hipStream_t s, Timing *tm)
 {
-    if (precision == RICK_FP64) return fft_impl<double>(p, in, out, tmp, scale, s);
-    return fft_impl<float>(p, in, out, tmp, scale, s);
+    Timing off;
+    if (tm == nullptr) tm = &off;
+    if (precision == RICK_FP64) return fft_impl<double>(p, in, out, tmp, scale, s, tm);
+    return fft_impl<float>(p, in, out, tmp, scale, s, tm);
 }
 
 }  // namespace rick

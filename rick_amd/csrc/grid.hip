@@ -443,15 +443,17 @@ k_tile(GridGeom g, const uint32_t *__restrict__ offsets, const uint32_t *__restr
 
 template <int PREC>
 hipError_t launch_tile(const GridGeom &g, const GridWork &w, const double *table, void *grid,
-                       int accumulate, hipStream_t s)
+                       int accumulate, hipStream_t s, Timing *tm)
 {
     using Rec = typename RecOf<PREC>::type;
     const Rec *rec = static_cast<const Rec *>(w.records);
     const dim3 grd(w.task_cap), blk(kThreads);
 #define RICK_TILE_CASE(A, B)                                                                 \
     if (g.tu == A && g.tv == B) {                                                            \
+        tm->begin(RICK_STAGE_TILE, s);                                                       \
         hipLaunchKernelGGL((k_tile<PREC, A, B>), grd, blk, 0, s, g, w.offsets, w.task_off,  \
                            w.meta, rec, table, grid, accumulate);                            \
+        tm->end(RICK_STAGE_TILE, s);                                                         \
         return hipGetLastError();                                                            \
     }
     RICK_TILE_CASE(64, 64)
@@ -524,32 +526,46 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
                        const double *table_d, uint32_t n, uint32_t F, uint32_t P,
                        const double *uu, const double *vv, const double *ww, const float *vr,
                        const float *vi, const float *wt, void *grid, int accumulate,
-                       hipStream_t s)
+                       hipStream_t s, Timing *tm)
 {
+    Timing off;
+    if (tm == nullptr) tm = &off;
     const uint32_t nblk = (n + kThreads - 1) / kThreads;
     const uint32_t fill_blocks = min((g.nb + 255u) / 256u, 4096u);
+    tm->begin(RICK_STAGE_COUNT_ROWS, s);
     hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks), dim3(256), 0, s, w.counts, g.nb, 0u);
     if (n > 0)
         hipLaunchKernelGGL(k_count, dim3(nblk), dim3(kThreads), 0, s, g, n, uu, vv, ww, w.counts);
+    tm->end(RICK_STAGE_COUNT_ROWS, s);
+    tm->begin(RICK_STAGE_SCAN, s);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, s, g, w.counts, w.offsets,
                        w.cursor, w.task_off, w.split_list, w.split_cap, w.meta);
+    tm->end(RICK_STAGE_SCAN, s);
     if (precision == RICK_FP64) {
+        tm->begin(RICK_STAGE_SCATTER, s);
         if (n > 0)
             hipLaunchKernelGGL(k_scatter<RICK_FP64>, dim3(nblk), dim3(kThreads), 0, s, g, n, F, P,
                                uu, vv, ww, vr, vi, wt, w.cursor,
                                static_cast<RecF64 *>(w.records));
+        tm->end(RICK_STAGE_SCATTER, s);
+        tm->begin(RICK_STAGE_ZERO, s);
         if (!accumulate)
             hipLaunchKernelGGL(k_zero_split<double>, dim3(w.split_cap), dim3(kThreads), 0, s, g,
                                w.split_list, w.meta, static_cast<double *>(grid));
-        return launch_tile<RICK_FP64>(g, w, table_d, grid, accumulate, s);
+        tm->end(RICK_STAGE_ZERO, s);
+        return launch_tile<RICK_FP64>(g, w, table_d, grid, accumulate, s, tm);
     }
+    tm->begin(RICK_STAGE_SCATTER, s);
     if (n > 0)
         hipLaunchKernelGGL(k_scatter<RICK_FP32>, dim3(nblk), dim3(kThreads), 0, s, g, n, F, P, uu,
                            vv, ww, vr, vi, wt, w.cursor, static_cast<RecF32 *>(w.records));
+    tm->end(RICK_STAGE_SCATTER, s);
+    tm->begin(RICK_STAGE_ZERO, s);
     if (!accumulate)
         hipLaunchKernelGGL(k_zero_split<float>, dim3(w.split_cap), dim3(kThreads), 0, s, g,
                            w.split_list, w.meta, static_cast<float *>(grid));
-    return launch_tile<RICK_FP32>(g, w, table_d, grid, accumulate, s);
+    tm->end(RICK_STAGE_ZERO, s);
+    return launch_tile<RICK_FP32>(g, w, table_d, grid, accumulate, s, tm);
 }
 
 }  // namespace rick

@@ -90,13 +90,15 @@ k_phase(PhaseGeom g, const void *__restrict__ gridss_v, double *__restrict__ ore
 }  // namespace
 
 hipError_t launch_phase(const PhaseGeom &g, int precision, const void *gridss, double *re,
-                        double *im, int accumulate, hipStream_t s)
+                        double *im, int accumulate, hipStream_t s, Timing *tm)
 {
     const dim3 grd((g.nx + kBlk - 1) / kBlk, g.ny), blk(kBlk);
+    if (tm) tm->begin(RICK_STAGE_PHASE, s);
     if (precision == RICK_FP64)
         hipLaunchKernelGGL(k_phase<RICK_FP64>, grd, blk, 0, s, g, gridss, re, im, accumulate);
     else
         hipLaunchKernelGGL(k_phase<RICK_FP32>, grd, blk, 0, s, g, gridss, re, im, accumulate);
+    if (tm) tm->end(RICK_STAGE_PHASE, s);
     return hipGetLastError();
 }
 

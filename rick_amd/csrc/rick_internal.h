@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <utility>
+#include <vector>
 
 #include "../../include/rick.h"
 
@@ -16,6 +18,31 @@ constexpr double kRefPi = 3.14159265359;
 // Maximum convolution support handled by the gridder.
 constexpr int kMaxSupport = 15;
 constexpr int kMaxTable = kOversample * kMaxSupport;
+
+// Optional per-stage kernel timing with HIP events recorded on the launch
+// stream (stage ids are the RICK_STAGE_* values of rick.h).
+struct Timing {
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[RICK_STAGE_COUNT];
+    size_t used[RICK_STAGE_COUNT] = {};
+    void begin(int st, hipStream_t s)
+    {
+        if (!on) return;
+        if (used[st] == ev[st].size()) {
+            hipEvent_t a = nullptr, b = nullptr;
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            ev[st].emplace_back(a, b);
+        }
+        (void)hipEventRecord(ev[st][used[st]].first, s);
+    }
+    void end(int st, hipStream_t s)
+    {
+        if (!on) return;
+        (void)hipEventRecord(ev[st][used[st]].second, s);
+        used[st]++;
+    }
+};
 
 // Geometry of one gridding problem.
 struct GridGeom {
@@ -51,7 +78,7 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
                        const double *table_d, uint32_t num_points, uint32_t F, uint32_t P,
                        const double *uu, const double *vv, const double *ww,
                        const float *vr, const float *vi, const float *wt, void *grid,
-                       int accumulate, hipStream_t s);
+                       int accumulate, hipStream_t s, Timing *tm);
 
 // FFT: per-plane 2-D backward transform of planes x ny x nx complex, scaled.
 struct FftPlan {
@@ -62,7 +89,7 @@ struct FftPlan {
 // in-place transform then stages each plane through tmp (one plane).
 bool fft_needs_tmp(int precision, int ny);
 hipError_t launch_fft(const FftPlan &p, int precision, const void *in, void *out, void *tmp,
-                      double scale, hipStream_t s);
+                      double scale, hipStream_t s, Timing *tm);
 
 // Phase correction + plane sum.
 struct PhaseGeom {
@@ -72,6 +99,6 @@ struct PhaseGeom {
     const double *ltab, *mtab;  // device: sin(xcoord*resolution) per u / per v index
 };
 hipError_t launch_phase(const PhaseGeom &p, int precision, const void *gridss, double *re,
-                        double *im, int accumulate, hipStream_t s);
+                        double *im, int accumulate, hipStream_t s, Timing *tm);
 
 }  // namespace rick

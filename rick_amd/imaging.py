@@ -151,6 +151,23 @@ class Plan:
         torch = _torch()
         return torch.float64 if self.precision == FP64 else torch.float32
 
+    def profile(self, enable: bool = True) -> None:
+        """Bracket every kernel this plan launches with HIP events."""
+        check(lib().rick_plan_profile(self._h, int(bool(enable))))
+
+    def stage_times(self) -> dict:
+        """{stage: (summed kernel ms, launches)} since the last query (waits)."""
+        n = len(_lib.STAGES)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int * n)()
+        check(lib().rick_plan_stage_times(self._h, ms, cnt, n))
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(_lib.STAGES)}
+
+    def grid_stats(self) -> dict:
+        s = (ctypes.c_uint * 3)()
+        check(lib().rick_plan_grid_stats(self._h, s, 3))
+        return {"records": s[0], "tasks": s[1], "split_buckets": s[2]}
+
     def new_grid(self):
         torch = _torch()
         return torch.empty(2 * self.planes * self.ny * self.nx, dtype=self.grid_dtype,
