@@ -4,22 +4,25 @@
 // plane-major grid layout.
 //
 // Structure:
-//   * a Stockham autosort engine: each thread holds 16 complex values in
-//     registers; a pass of radix R (16, 8, 4 or 2) reads R values, applies the
-//     inter-pass twiddles, does an in-register DFT and writes back through LDS;
-//     the first pass reads global memory and the last writes global memory.
-//   * k_rows: lines are grid rows (contiguous u), G rows per workgroup.
-//   * k_cols: lines are columns of a W-column strip; a line of length L has
-//     element n at row  row_base + n*row_stride  and is written to row
-//     out_base + n*out_stride, optionally multiplied by W_N^(n*tw_mult).  With
-//     L = Ny this is the whole column transform; for long columns the
-//     transform is split into two such passes (N = N1*N2, four-step):
+//   * a Stockham autosort engine whose pass plan (radix 16 passes, then one
+//     pass of radix 2/4/8) is fixed at compile time by the line length L.
+//     Each thread holds 16 complex values in registers; a pass reads R values,
+//     applies the inter-pass twiddles, does an in-register DFT and writes back
+//     through LDS; the first pass reads global memory, the last writes it.
+//   * k_fft_rows: lines are grid rows (contiguous u), 4096/L rows per
+//     workgroup.
+//   * k_fft_cols: lines are the columns of a G-column strip; element n of a
+//     line is at row  s*in_set_mul + n*in_stride  and goes to row
+//     s*out_set_mul + n*out_stride, optionally multiplied by W_N^(n*s).  With
+//     L = Ny this is the whole column transform; long columns use two such
+//     passes (N = N1*N2, four-step):
 //       A: N1-point DFTs over rows n2 + N2*n1, twiddle W_N^(k1*n2),
 //          written to rows n2*N1 + k1 (out of place);
 //       B: N2-point DFTs over rows k1 + N1*n2, written back in place, which
 //          leaves X[k1 + N1*k2] at row k1 + N1*k2 (natural order).
 // Twiddle tables hold exp(+2*pi*i*m/N) for the axis length N; an L-point
-// sub-transform uses them with stride N/L.
+// sub-transform uses them with stride N/L.  Per butterfly only W^k, W^2k,
+// W^4k, W^8k are loaded; the other powers are products of those.
 #include "rick_internal.h"
 
 #include <algorithm>
@@ -47,41 +50,24 @@ __device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b)
 template <typename T>
 __device__ __forceinline__ C2<T> mul_i(C2<T> a) { return {-a.y, a.x}; }
 
-// exp(+2*pi*i*m/16), m = 0..15, as literals.
-__device__ __forceinline__ double tw16_re(int m)
+// exp(+2*pi*i*m/16) with m known at compile time.
+template <typename T, int M>
+__device__ __forceinline__ C2<T> rot16(C2<T> a)
 {
-    constexpr double c[16] = {1.0, 0.92387953251128675613, 0.70710678118654752440,
-                              0.38268343236508977173, 0.0, -0.38268343236508977173,
-                              -0.70710678118654752440, -0.92387953251128675613, -1.0,
-                              -0.92387953251128675613, -0.70710678118654752440,
-                              -0.38268343236508977173, 0.0, 0.38268343236508977173,
-                              0.70710678118654752440, 0.92387953251128675613};
-    return c[m & 15];
-}
-__device__ __forceinline__ double tw16_im(int m) { return tw16_re(m - 4); }
-
-// Multiply by exp(+2*pi*i*m/16) with m a compile-time constant after
-// unrolling; the trivial cases avoid multiplications.
-template <typename T>
-__device__ __forceinline__ C2<T> rot16(C2<T> a, int m)
-{
-    m &= 15;
-    if (m == 0) return a;
-    if (m == 4) return mul_i(a);
-    if (m == 8) return {-a.x, -a.y};
-    if (m == 12) return {a.y, -a.x};
-    const C2<T> w = {(T)tw16_re(m), (T)tw16_im(m)};
-    return cmul(a, w);
+    constexpr int m = M & 15;
+    constexpr double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173,
+                     h = 0.70710678118654752440;
+    if constexpr (m == 0) return a;
+    else if constexpr (m == 4) return mul_i(a);
+    else if constexpr (m == 8) return {-a.x, -a.y};
+    else if constexpr (m == 12) return {a.y, -a.x};
+    else {
+        constexpr double tr[16] = {1, c1, h, s1, 0, -s1, -h, -c1, -1, -c1, -h, -s1, 0, s1, h, c1};
+        constexpr double re = tr[m], im = tr[(m + 12) & 15];
+        return cmul(a, C2<T>{(T)re, (T)im});
+    }
 }
 
-// In-register DFT with sign +1: X[k] = sum_n x[n] exp(+2 pi i n k / R).
-template <typename T>
-__device__ __forceinline__ void dft2(C2<T> *v)
-{
-    const C2<T> a = v[0], b = v[1];
-    v[0] = cadd(a, b);
-    v[1] = csub(a, b);
-}
 template <typename T>
 __device__ __forceinline__ void dft4(C2<T> &x0, C2<T> &x1, C2<T> &x2, C2<T> &x3)
 {
@@ -91,243 +77,272 @@ __device__ __forceinline__ void dft4(C2<T> &x0, C2<T> &x1, C2<T> &x2, C2<T> &x3)
     x1 = cadd(b, d);
     x3 = csub(b, d);
 }
-template <typename T>
-__device__ __forceinline__ void dft4(C2<T> *v) { dft4(v[0], v[1], v[2], v[3]); }
-template <typename T>
-__device__ __forceinline__ void dft8(C2<T> *v)
-{
-    // n = 2*n1 + n2, k = k1 + 4*k2.
-    C2<T> y[2][4];
-#pragma unroll
-    for (int n2 = 0; n2 < 2; ++n2) {
-#pragma unroll
-        for (int n1 = 0; n1 < 4; ++n1) y[n2][n1] = v[2 * n1 + n2];
-        dft4(y[n2]);
-#pragma unroll
-        for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = rot16(y[n2][k1], 2 * n2 * k1);
-    }
-#pragma unroll
-    for (int k1 = 0; k1 < 4; ++k1) {
-        v[k1] = cadd(y[0][k1], y[1][k1]);
-        v[k1 + 4] = csub(y[0][k1], y[1][k1]);
-    }
-}
-template <typename T>
-__device__ __forceinline__ void dft16(C2<T> *v)
-{
-    // n = 4*n1 + n2, k = k1 + 4*k2.
-    C2<T> y[4][4];
-#pragma unroll
-    for (int n2 = 0; n2 < 4; ++n2) {
-#pragma unroll
-        for (int n1 = 0; n1 < 4; ++n1) y[n2][n1] = v[4 * n1 + n2];
-        dft4(y[n2]);
-#pragma unroll
-        for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = rot16(y[n2][k1], n2 * k1);
-    }
-#pragma unroll
-    for (int k1 = 0; k1 < 4; ++k1) {
-        C2<T> z0 = y[0][k1], z1 = y[1][k1], z2 = y[2][k1], z3 = y[3][k1];
-        dft4(z0, z1, z2, z3);
-        v[k1] = z0;
-        v[k1 + 4] = z1;
-        v[k1 + 8] = z2;
-        v[k1 + 12] = z3;
-    }
-}
+
+// In-register DFT with sign +1: X[k] = sum_n x[n] exp(+2 pi i n k / R).
 template <typename T, int R>
 __device__ __forceinline__ void dftR(C2<T> *v)
 {
-    if constexpr (R == 2) dft2(v);
-    else if constexpr (R == 4) dft4(v);
-    else if constexpr (R == 8) dft8(v);
-    else dft16(v);
+    if constexpr (R == 2) {
+        const C2<T> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        dft4(v[0], v[1], v[2], v[3]);
+    } else if constexpr (R == 8) {
+        // n = 2*n1 + n2, k = k1 + 4*k2.
+        C2<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+        C2<T> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+        dft4(e0, e1, e2, e3);
+        dft4(o0, o1, o2, o3);
+        o1 = rot16<T, 2>(o1);
+        o2 = rot16<T, 4>(o2);
+        o3 = rot16<T, 6>(o3);
+        v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+        v[1] = cadd(e1, o1); v[5] = csub(e1, o1);
+        v[2] = cadd(e2, o2); v[6] = csub(e2, o2);
+        v[3] = cadd(e3, o3); v[7] = csub(e3, o3);
+    } else {
+        // n = 4*n1 + n2, k = k1 + 4*k2.
+        C2<T> y[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) {
+            y[n2][0] = v[n2];
+            y[n2][1] = v[4 + n2];
+            y[n2][2] = v[8 + n2];
+            y[n2][3] = v[12 + n2];
+            dft4(y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
+        }
+        y[1][1] = rot16<T, 1>(y[1][1]); y[1][2] = rot16<T, 2>(y[1][2]); y[1][3] = rot16<T, 3>(y[1][3]);
+        y[2][1] = rot16<T, 2>(y[2][1]); y[2][2] = rot16<T, 4>(y[2][2]); y[2][3] = rot16<T, 6>(y[2][3]);
+        y[3][1] = rot16<T, 3>(y[3][1]); y[3][2] = rot16<T, 6>(y[3][2]); y[3][3] = rot16<T, 9>(y[3][3]);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            dft4(y[0][k1], y[1][k1], y[2][k1], y[3][k1]);
+            v[k1] = y[0][k1];
+            v[k1 + 4] = y[1][k1];
+            v[k1 + 8] = y[2][k1];
+            v[k1 + 12] = y[3][k1];
+        }
+    }
 }
 
-__host__ __device__ __forceinline__ int lpad(int n) { return n + (n >> 4); }
+// Compile-time pass plan of an L-point transform.
+constexpr int n16(int L)
+{
+    int n = 0;
+    while (L >= 16 && L % 16 == 0) { L /= 16; ++n; }
+    return n;
+}
+constexpr int pow16(int n) { return n == 0 ? 1 : 16 * pow16(n - 1); }
+constexpr int npasses(int L) { return n16(L) + (L / pow16(n16(L)) > 1 ? 1 : 0); }
+constexpr int radix_of(int L, int p) { return p < n16(L) ? 16 : L / pow16(n16(L)); }
+constexpr int ns_of(int L, int p) { return p == 0 ? 1 : ns_of(L, p - 1) * radix_of(L, p - 1); }
 
-// Line layouts.  Rows: line g element n at LDS g*LP + lpad(n), butterfly b ->
-// (g = b / (L/R), j = b % (L/R)).  Columns: element (n, c) at n*W + c,
-// butterfly b -> (c = b % W, j = b / W).
-struct RowMap {
-    int L, LP;
-    __device__ int lds(int g, int n) const { return g * LP + lpad(n); }
-};
-struct ColMap {
-    int L, W;
-    __device__ int lds(int c, int n) const { return n * W + c; }
-};
+__host__ __device__ constexpr int lpad(int n) { return n + (n >> 4); }
 
-// One radix-R pass over the data held by this workgroup.  Values come from
-// `src` (registers already loaded) and leave through LDS or registers; this
-// helper only does twiddle + DFT on one butterfly.
+// Twiddles W^(r*k*step), r = 1..R-1, from four loads.
 template <typename T, int R>
-__device__ __forceinline__ void butterfly(C2<T> *v, int j, int Ns, int L, int ts,
-                                          const C2<T> *__restrict__ tw)
+__device__ __forceinline__ void twiddle(C2<T> *v, int k, int step, const C2<T> *__restrict__ tw)
 {
-    if (Ns > 1) {
-        const int k = j % Ns;
-        const int step = (L / (Ns * R)) * ts;
-#pragma unroll
-        for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[(r * k) * step]);
+    const C2<T> w1 = tw[k * step];
+    v[1] = cmul(v[1], w1);
+    if constexpr (R >= 4) {
+        const C2<T> w2 = tw[2 * k * step];
+        const C2<T> w3 = cmul(w1, w2);
+        v[2] = cmul(v[2], w2);
+        v[3] = cmul(v[3], w3);
+        if constexpr (R >= 8) {
+            const C2<T> w4 = tw[4 * k * step];
+            v[4] = cmul(v[4], w4);
+            v[5] = cmul(v[5], cmul(w1, w4));
+            v[6] = cmul(v[6], cmul(w2, w4));
+            const C2<T> w7 = cmul(w3, w4);
+            v[7] = cmul(v[7], w7);
+            if constexpr (R == 16) {
+                const C2<T> w8 = tw[8 * k * step];
+                v[8] = cmul(v[8], w8);
+                v[9] = cmul(v[9], cmul(w1, w8));
+                v[10] = cmul(v[10], cmul(w2, w8));
+                v[11] = cmul(v[11], cmul(w3, w8));
+                const C2<T> w12 = cmul(w4, w8);
+                v[12] = cmul(v[12], w12);
+                v[13] = cmul(v[13], cmul(w1, w12));
+                v[14] = cmul(v[14], cmul(w2, w12));
+                v[15] = cmul(v[15], cmul(w7, w8));
+            }
+        }
     }
-    dftR<T, R>(v);
 }
 
-struct Passes {
-    int np;
-    int r[6];
+// Addressing of one workgroup's lines.
+struct Lines {
+    size_t in_base, out_base;   // element offsets of line 0
+    size_t in_line, out_line;   // offset between consecutive lines
+    size_t in_el, out_el;       // offset between consecutive elements of a line
+    int nvalid;                 // lines of this workgroup that exist
+    int tw_set;                 // twiddle W_N^(n*tw_set) after the last pass (0: none)
 };
 
-__host__ Passes plan_passes(int L)
+// Butterfly b of a pass with L/R butterflies per line: rows map consecutive
+// b to consecutive j of one line, columns to consecutive lines.
+template <bool ROWS, int LR, int G>
+__device__ __forceinline__ void bmap(int b, int &g, int &j)
 {
-    Passes p{};
-    int rest = L;
-    while (rest >= 16 && (rest % 16) == 0) { p.r[p.np++] = 16; rest /= 16; }
-    if (rest > 1) p.r[p.np++] = rest;  // 2, 4 or 8
-    return p;
+    if constexpr (ROWS) { g = b / LR; j = b % LR; }
+    else { g = b % G; j = b / G; }
 }
 
-// Generic driver: Layout supplies global addressing for (line, n).
-template <typename T, int R, bool ROWS>
-__device__ __forceinline__ void do_pass(C2<T> *vals, int tid, int nthreads, int L, int G, int Ns,
-                                        int ts, const C2<T> *tw, bool from_lds, bool to_lds,
-                                        C2<T> *lds, int LP, int W,
-                                        const C2<T> *gin, C2<T> *gout,
-                                        size_t line_base0, size_t line_step, size_t estride,
-                                        size_t out_base0, size_t out_step, size_t ostride,
-                                        int tw_mult, int twN_stride, T scale)
+template <typename T, int L, int G, bool ROWS, int P>
+__device__ __forceinline__ void pass(C2<T> *vals, int tid, const Lines &ln, int ts,
+                                     const C2<T> *__restrict__ tw, C2<T> *lds, const C2<T> *gin,
+                                     C2<T> *gout, T scale)
 {
-    // butterflies per thread
-    constexpr int BPT = kE / R;
-    const int LR = L / R;
+    constexpr int NP = npasses(L);
+    if constexpr (P < NP) {
+        constexpr int R = radix_of(L, P), Ns = ns_of(L, P), LR = L / R, BPT = kE / R;
+        constexpr int NT = G * L / kE;
+        constexpr int LP = lpad(L) + 1;
+        constexpr bool from_lds = P > 0, to_lds = P + 1 < NP;
 #pragma unroll
-    for (int i = 0; i < BPT; ++i) {
-        const int b = tid + i * nthreads;
-        int g, j;
-        if (ROWS) { g = b / LR; j = b % LR; }
-        else { g = b % W; j = b / W; }
-        C2<T> *v = vals + i * R;
+        for (int i = 0; i < BPT; ++i) {
+            int g, j;
+            bmap<ROWS, LR, G>(tid + i * NT, g, j);
+            C2<T> *v = vals + i * R;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int n = j + r * LR;
-            if (from_lds) {
-                v[r] = ROWS ? lds[g * LP + lpad(n)] : lds[n * W + g];
-            } else {
-                v[r] = gin[line_base0 + (size_t)g * line_step + (size_t)n * estride];
+            for (int r = 0; r < R; ++r) {
+                const int n = j + r * LR;
+                if constexpr (from_lds) {
+                    v[r] = ROWS ? lds[g * LP + lpad(n)] : lds[n * G + g];
+                } else {
+                    v[r] = g < ln.nvalid ? gin[ln.in_base + (size_t)g * ln.in_line +
+                                               (size_t)n * ln.in_el]
+                                         : C2<T>{T(0), T(0)};
+                }
             }
         }
-    }
-    if (from_lds) __syncthreads();
+        if constexpr (from_lds) __syncthreads();
 #pragma unroll
-    for (int i = 0; i < BPT; ++i) {
-        const int b = tid + i * nthreads;
-        const int j = ROWS ? b % LR : b / W;
-        butterfly<T, R>(vals + i * R, j, Ns, L, ts, tw);
-    }
+        for (int i = 0; i < BPT; ++i) {
+            int g, j;
+            bmap<ROWS, LR, G>(tid + i * NT, g, j);
+            C2<T> *v = vals + i * R;
+            if constexpr (Ns > 1) twiddle<T, R>(v, j % Ns, (L / (Ns * R)) * ts, tw);
+            dftR<T, R>(v);
+            const int dbase = (j / Ns) * Ns * R + j % Ns;
 #pragma unroll
-    for (int i = 0; i < BPT; ++i) {
-        const int b = tid + i * nthreads;
-        int g, j;
-        if (ROWS) { g = b / LR; j = b % LR; }
-        else { g = b % W; j = b / W; }
-        C2<T> *v = vals + i * R;
-        const int k = j % Ns;
-        const int dbase = (j / Ns) * Ns * R + k;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int n = dbase + r * Ns;
-            if (to_lds) {
-                if (ROWS) lds[g * LP + lpad(n)] = v[r];
-                else lds[n * W + g] = v[r];
-            } else {
-                C2<T> o = v[r];
-                if (tw_mult) o = cmul(o, tw[(size_t)((n * tw_mult) * twN_stride)]);
-                o.x *= scale;
-                o.y *= scale;
-                gout[out_base0 + (size_t)g * out_step + (size_t)n * ostride] = o;
+            for (int r = 0; r < R; ++r) {
+                const int n = dbase + r * Ns;
+                if constexpr (to_lds) {
+                    if (ROWS) lds[g * LP + lpad(n)] = v[r];
+                    else lds[n * G + g] = v[r];
+                } else {
+                    C2<T> o = v[r];
+                    if (ln.tw_set) o = cmul(o, tw[(size_t)n * ln.tw_set]);
+                    o.x *= scale;
+                    o.y *= scale;
+                    if (g < ln.nvalid)
+                        gout[ln.out_base + (size_t)g * ln.out_line + (size_t)n * ln.out_el] = o;
+                }
             }
         }
-    }
-    if (to_lds) __syncthreads();
-}
-
-template <typename T, bool ROWS>
-__device__ void run_passes(const Passes &P, int tid, int nthreads, int L, int G, int ts,
-                           const C2<T> *tw, C2<T> *lds, int LP, int W, const C2<T> *gin,
-                           C2<T> *gout, size_t line_base0, size_t line_step, size_t estride,
-                           size_t out_base0, size_t out_step, size_t ostride, int tw_mult,
-                           int twN_stride, T scale)
-{
-    C2<T> vals[kE];
-    int Ns = 1;
-    for (int p = 0; p < P.np; ++p) {
-        const bool from_lds = p > 0, to_lds = p + 1 < P.np;
-        const int R = P.r[p];
-#define RICK_PASS(RR)                                                                        \
-    do_pass<T, RR, ROWS>(vals, tid, nthreads, L, G, Ns, ts, tw, from_lds, to_lds, lds, LP, W, \
-                         gin, gout, line_base0, line_step, estride, out_base0, out_step,    \
-                         ostride, tw_mult, twN_stride, scale)
-        if (R == 16) RICK_PASS(16);
-        else if (R == 8) RICK_PASS(8);
-        else if (R == 4) RICK_PASS(4);
-        else RICK_PASS(2);
-#undef RICK_PASS
-        Ns *= R;
+        if constexpr (to_lds) __syncthreads();
+        pass<T, L, G, ROWS, P + 1>(vals, tid, ln, ts, tw, lds, gin, gout, scale);
     }
 }
 
-// Rows: grid of (planes * ny / G) workgroups of L*G/16 threads.
-template <typename T>
-__global__ void __launch_bounds__(512)
-k_rows(Passes P, int L, int G, int rows_total, int ts, const C2<T> *__restrict__ tw,
-       const C2<T> *gin, C2<T> *gout, T scale)
+// Lines per workgroup: 4096 elements per workgroup (256 threads), or one
+// line of 8192 (512 threads).
+constexpr int lines_per_wg(int L) { return L >= 4096 ? 1 : 4096 / L; }
+constexpr int lds_elems(int L, bool rows)
 {
-    extern __shared__ unsigned char smem_raw[];
-    C2<T> *lds = reinterpret_cast<C2<T> *>(smem_raw);
-    const int LP = lpad(L) + 1;
+    return rows ? lines_per_wg(L) * (lpad(L) + 1) : lines_per_wg(L) * L;
+}
+
+// Rows: workgroup w handles rows w*G .. w*G+G-1 of the planes*ny rows.
+template <typename T, int L>
+__global__ void __launch_bounds__(L >= 8192 ? 512 : 256)
+k_fft_rows(int rows_total, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gout, T scale)
+{
+    constexpr int G = lines_per_wg(L);
+    __shared__ C2<T> lds[lds_elems(L, true)];
+    Lines ln;
     const size_t row0 = (size_t)blockIdx.x * G;
-    const size_t base = row0 * (size_t)L;
-    (void)rows_total;
-    run_passes<T, true>(P, threadIdx.x, blockDim.x, L, G, ts, tw, lds, LP, 0, gin, gout, base,
-                        (size_t)L, 1, base, (size_t)L, 1, 0, 0, scale);
+    ln.in_base = ln.out_base = row0 * L;
+    ln.in_line = ln.out_line = L;
+    ln.in_el = ln.out_el = 1;
+    ln.nvalid = (int)min((size_t)G, (size_t)rows_total - row0);
+    ln.tw_set = 0;
+    C2<T> vals[kE];
+    pass<T, L, G, true, 0>(vals, threadIdx.x, ln, 1, tw, lds, gin, gout, scale);
 }
 
-// Columns: workgroup (x: strip of W columns, y: row-set index s, z: plane).
-// Line element n of column c sits at row  rb + n*rs  with rb = s*sb_in + s2,
-// see the file comment; output row  ob + n*os.
+// Columns: workgroup (x: strip of G columns, y: row set s, z: plane).
 struct ColArgs {
-    int L, W, nx, ny;
+    int nx, ny;
     int in_set_mul, in_stride;     // input row  = s*in_set_mul + n*in_stride
     int out_set_mul, out_stride;   // output row = s*out_set_mul + n*out_stride
     int tw_by_set;                 // twiddle W_N^(n*s) after the DFT (four-step A)
     int ts;                        // table stride for the L-point DFT
-    int twN_stride;                // table stride for W_N (N = ny)
 };
 
-template <typename T>
-__global__ void __launch_bounds__(512)
-k_cols(Passes P, ColArgs a, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gout, T scale)
+template <typename T, int L>
+__global__ void __launch_bounds__(L >= 8192 ? 512 : 256)
+k_fft_cols(ColArgs a, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gout, T scale)
 {
-    extern __shared__ unsigned char smem_raw[];
-    C2<T> *lds = reinterpret_cast<C2<T> *>(smem_raw);
-    const int c0 = blockIdx.x * a.W;
+    constexpr int G = lines_per_wg(L);
+    __shared__ C2<T> lds[lds_elems(L, false)];
+    const int c0 = blockIdx.x * G;
     const int s = blockIdx.y;
     const size_t plane = (size_t)blockIdx.z * (size_t)a.nx * (size_t)a.ny;
-    const size_t in_base = plane + (size_t)s * a.in_set_mul * (size_t)a.nx + c0;
-    const size_t out_base = plane + (size_t)s * a.out_set_mul * (size_t)a.nx + c0;
-    run_passes<T, false>(P, threadIdx.x, blockDim.x, a.L, a.W, a.ts, tw, lds, 0, a.W, gin, gout,
-                         in_base, 1, (size_t)a.in_stride * a.nx, out_base, 1,
-                         (size_t)a.out_stride * a.nx, a.tw_by_set ? s : 0, a.twN_stride, scale);
+    Lines ln;
+    ln.in_base = plane + (size_t)s * a.in_set_mul * (size_t)a.nx + c0;
+    ln.out_base = plane + (size_t)s * a.out_set_mul * (size_t)a.nx + c0;
+    ln.in_line = ln.out_line = 1;
+    ln.in_el = (size_t)a.in_stride * a.nx;
+    ln.out_el = (size_t)a.out_stride * a.nx;
+    ln.nvalid = min(G, a.nx - c0);
+    ln.tw_set = a.tw_by_set ? s : 0;
+    C2<T> vals[kE];
+    pass<T, L, G, false, 0>(vals, threadIdx.x, ln, a.ts, tw, lds, gin, gout, scale);
 }
 
-// Kernels using more than 64 KB of dynamic LDS must say so before launch.
-template <typename K>
-hipError_t allow_lds(K kernel, size_t bytes)
+// Launch helpers dispatching on the (power-of-two) line length.
+template <typename T>
+hipError_t launch_rows(int L, int rows_total, const C2<T> *tw, const C2<T> *in, C2<T> *out,
+                       T scale, hipStream_t s)
 {
-    if (bytes <= 65536) return hipSuccess;
-    return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+#define RICK_ROWS(LL)                                                                      \
+    case LL: {                                                                             \
+        constexpr int G = lines_per_wg(LL);                                                \
+        hipLaunchKernelGGL((k_fft_rows<T, LL>), dim3((rows_total + G - 1) / G),            \
+                           dim3(G * LL / kE), 0, s, rows_total, tw, in, out, scale);       \
+        return hipGetLastError();                                                          \
+    }
+    switch (L) {
+        RICK_ROWS(16) RICK_ROWS(32) RICK_ROWS(64) RICK_ROWS(128) RICK_ROWS(256)
+        RICK_ROWS(512) RICK_ROWS(1024) RICK_ROWS(2048) RICK_ROWS(4096) RICK_ROWS(8192)
+        default: return hipErrorInvalidValue;
+    }
+#undef RICK_ROWS
+}
+
+template <typename T>
+hipError_t launch_cols(int L, const ColArgs &a, int sets, int planes, const C2<T> *tw,
+                       const C2<T> *in, C2<T> *out, hipStream_t s)
+{
+#define RICK_COLS(LL)                                                                        \
+    case LL: {                                                                               \
+        constexpr int G = lines_per_wg(LL);                                                  \
+        hipLaunchKernelGGL((k_fft_cols<T, LL>), dim3((a.nx + G - 1) / G, sets, planes),      \
+                           dim3(G * LL / kE), 0, s, a, tw, in, out, T(1));                   \
+        return hipGetLastError();                                                            \
+    }
+    switch (L) {
+        RICK_COLS(16) RICK_COLS(32) RICK_COLS(64) RICK_COLS(128) RICK_COLS(256)
+        RICK_COLS(512)
+        default: return hipErrorInvalidValue;
+    }
+#undef RICK_COLS
 }
 
 template <typename T>
@@ -340,113 +355,51 @@ hipError_t fft_impl(const FftPlan &p, const void *in_v, void *out_v, void *tmp_v
     const CT *twx = static_cast<const CT *>(p.tw_x);
     const CT *twy = static_cast<const CT *>(p.tw_y);
     const int nx = p.nx, ny = p.ny, planes = p.planes;
-    const T one = T(1), scale = (T)scale_d;
-    // Elements per workgroup: 4096, or the line length if longer.
-    const int chunk = 4096;
+    hipError_t e = hipSuccess;
 
     // ---- columns (v axis, length ny) ----
-    const CT *src = in;
-    if (ny > 1) {
-        if (!fft_needs_tmp(sizeof(T) == 8 ? RICK_FP64 : RICK_FP32, ny)) {
-            ColArgs a{};
-            a.L = ny;
-            a.W = min(nx, max(1, chunk / ny));
-            a.nx = nx;
-            a.ny = ny;
-            a.in_set_mul = 0;
-            a.in_stride = 1;
-            a.out_set_mul = 0;
-            a.out_stride = 1;
-            a.tw_by_set = 0;
-            a.ts = 1;
-            a.twN_stride = 1;
-            if (nx % a.W) return hipErrorInvalidValue;
-            const int threads = a.L * a.W / kE;
-            const size_t lds = sizeof(CT) * (size_t)a.L * a.W;
-            if (hipError_t e = allow_lds(k_cols<T>, lds); e != hipSuccess) return e;
+    if (!fft_needs_tmp(sizeof(T) == 8 ? RICK_FP64 : RICK_FP32, ny)) {
+        const ColArgs a{nx, ny, 0, 1, 0, 1, 0, 1};
+        tm->begin(RICK_STAGE_FFT_COLS, s);
+        e = launch_cols<T>(ny, a, 1, planes, twy, in, out, s);
+        tm->end(RICK_STAGE_FFT_COLS, s);
+        if (e != hipSuccess) return e;
+    } else {
+        // Four-step: ny = N1 * N2 with N1 = 64.
+        const int N1 = 64, N2 = ny / 64;
+        const ColArgs A{nx, ny, 1, N2, N1, 1, 1, N2};   // rows n2 + n1*N2 -> n2*N1 + k1
+        const ColArgs B{nx, ny, 1, N1, 1, N1, 0, N1};   // rows k1 + n2*N1 -> k1 + k2*N1
+        if (in_v != out_v) {
             tm->begin(RICK_STAGE_FFT_COLS, s);
-            hipLaunchKernelGGL(k_cols<T>, dim3(nx / a.W, 1, planes), dim3(threads), lds, s,
-                               plan_passes(a.L), a, twy, src, out, one);
+            e = launch_cols<T>(N1, A, N2, planes, twy, in, out, s);
             tm->end(RICK_STAGE_FFT_COLS, s);
+            if (e != hipSuccess) return e;
+            tm->begin(RICK_STAGE_FFT_COLS_B, s);
+            e = launch_cols<T>(N2, B, N1, planes, twy, out, out, s);
+            tm->end(RICK_STAGE_FFT_COLS_B, s);
+            if (e != hipSuccess) return e;
         } else {
-            // Four-step: ny = N1 * N2 with N1 = 64.
-            const int N1 = 64, N2 = ny / 64;
-            ColArgs A{};
-            A.L = N1;
-            A.W = min(nx, chunk / N1);
-            A.nx = nx;
-            A.ny = ny;
-            A.in_set_mul = 1;        // row = n2 + n1*N2
-            A.in_stride = N2;
-            A.out_set_mul = N1;      // row = n2*N1 + k1
-            A.out_stride = 1;
-            A.tw_by_set = 1;         // W_N^(k1*n2)
-            A.ts = N2;
-            A.twN_stride = 1;
-            ColArgs B{};
-            B.L = N2;
-            B.W = min(nx, max(1, chunk / N2));
-            B.nx = nx;
-            B.ny = ny;
-            B.in_set_mul = 1;        // row = k1 + n2*N1
-            B.in_stride = N1;
-            B.out_set_mul = 1;       // row = k1 + k2*N1
-            B.out_stride = N1;
-            B.tw_by_set = 0;
-            B.ts = N1;
-            B.twN_stride = 1;
-            if (nx % A.W || nx % B.W) return hipErrorInvalidValue;
-            const size_t ldsA = sizeof(CT) * (size_t)A.L * A.W;
-            const size_t ldsB = sizeof(CT) * (size_t)B.L * B.W;
-            const Passes PA = plan_passes(A.L), PB = plan_passes(B.L);
-            if (hipError_t e = allow_lds(k_cols<T>, std::max(ldsA, ldsB)); e != hipSuccess)
-                return e;
-            if (in_v != out_v) {
-                // A: in -> out (rows permuted), B: in place on out.
+            // In place: per plane, A: plane -> tmp, B: tmp -> plane.
+            if (tmp_v == nullptr) return hipErrorInvalidValue;
+            CT *tmp = static_cast<CT *>(tmp_v);
+            const size_t pl = (size_t)nx * ny;
+            for (int iw = 0; iw < planes; ++iw) {
                 tm->begin(RICK_STAGE_FFT_COLS, s);
-                hipLaunchKernelGGL(k_cols<T>, dim3(nx / A.W, N2, planes), dim3(A.L * A.W / kE),
-                                   ldsA, s, PA, A, twy, src, out, one);
+                e = launch_cols<T>(N1, A, N2, 1, twy, out + iw * pl, tmp, s);
                 tm->end(RICK_STAGE_FFT_COLS, s);
+                if (e != hipSuccess) return e;
                 tm->begin(RICK_STAGE_FFT_COLS_B, s);
-                hipLaunchKernelGGL(k_cols<T>, dim3(nx / B.W, N1, planes), dim3(B.L * B.W / kE),
-                                   ldsB, s, PB, B, twy, out, out, one);
+                e = launch_cols<T>(N2, B, N1, 1, twy, tmp, out + iw * pl, s);
                 tm->end(RICK_STAGE_FFT_COLS_B, s);
-            } else {
-                // In place: per plane, A: plane -> tmp, B: tmp -> plane.
-                if (tmp_v == nullptr) return hipErrorInvalidValue;
-                CT *tmp = static_cast<CT *>(tmp_v);
-                const size_t pl = (size_t)nx * ny;
-                for (int iw = 0; iw < planes; ++iw) {
-                    tm->begin(RICK_STAGE_FFT_COLS, s);
-                    hipLaunchKernelGGL(k_cols<T>, dim3(nx / A.W, N2, 1), dim3(A.L * A.W / kE),
-                                       ldsA, s, PA, A, twy, out + iw * pl, tmp, one);
-                    tm->end(RICK_STAGE_FFT_COLS, s);
-                    tm->begin(RICK_STAGE_FFT_COLS_B, s);
-                    hipLaunchKernelGGL(k_cols<T>, dim3(nx / B.W, N1, 1), dim3(B.L * B.W / kE),
-                                       ldsB, s, PB, B, twy, tmp, out + iw * pl, one);
-                    tm->end(RICK_STAGE_FFT_COLS_B, s);
-                }
+                if (e != hipSuccess) return e;
             }
         }
-        src = out;
     }
     // ---- rows (u axis, length nx), in place on out, scaled ----
-    if (nx > 1) {
-        const int G = min(ny, max(1, chunk / nx));
-        const int rows_total = planes * ny;
-        if (rows_total % G) return hipErrorInvalidValue;
-        const int threads = nx * G / kE;
-        const int LP = lpad(nx) + 1;
-        const size_t lds = sizeof(CT) * (size_t)LP * G;
-        if (hipError_t e = allow_lds(k_rows<T>, lds); e != hipSuccess) return e;
-        tm->begin(RICK_STAGE_FFT_ROWS, s);
-        hipLaunchKernelGGL(k_rows<T>, dim3(rows_total / G), dim3(threads), lds, s,
-                           plan_passes(nx), nx, G, rows_total, 1, twx, src, out, scale);
-        tm->end(RICK_STAGE_FFT_ROWS, s);
-    } else if (src != out || scale != one) {
-        return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    tm->begin(RICK_STAGE_FFT_ROWS, s);
+    e = launch_rows<T>(nx, planes * ny, twx, out, out, (T)scale_d, s);
+    tm->end(RICK_STAGE_FFT_ROWS, s);
+    return e;
 }
 
 }  // namespace

@@ -4,35 +4,47 @@
 //   grid[w][k][j] += sum_rows sum_{f,p} fl32(weight*vis) * K[jKer] * K[kKer]
 // over each row's (2KL+1)^2 footprint, with the same index arithmetic
 // (IEEE divides, truncation toward zero, +0.5 half-cell offsets, the
-// off-centre fabs(u_dist+KL) table look-up), but organised for the GPU:
+// off-centre fabs(u_dist+KL) table look-up), organised for the GPU:
 //
-//   k_count    one thread per row: w-plane and footprint -> up to 4 buckets
-//              (bucket = w-plane x TUxTV grid tile whose core the footprint
-//              touches); wave-aggregated counter increments.
-//   k_scan     one workgroup: exclusive scans -> bucket offsets, task offsets
-//              (a task is <= rmax rows of one bucket), list of split buckets.
-//   k_scatter  one thread per row: the channel/polarization sum
-//              S = sum_{f,p} fl32(weight*vis) (skip NaN real parts,
-//              w-stacking.cu:517) is formed once per row and a compact
-//              record {pos_u, pos_v, S} is written to each of its buckets.
-//   k_zero_split  clears the tiles of split buckets (overwrite mode).
-//   k_tile     one workgroup per task: accumulates its rows into an LDS tile
-//              (one lane per footprint cell, LDS atomics), then writes the
-//              tile once with plain, coalesced stores; tasks of split buckets
-//              flush with HBM atomics instead.  Every grid cell is written by
-//              exactly one tile, so no prior memset of the grid is needed.
+//   k_count      rows -> buckets (bucket = w-plane x TUxTV grid tile whose core
+//                the row's footprint touches; at most 4 per row).  Each
+//                workgroup aggregates its 1024 rows' keys in an LDS hash table
+//                and adds one count per distinct key to the global histogram,
+//                so hot central tiles see one global atomic per workgroup.
+//   k_scan_*     multi-block exclusive scans of the histogram -> bucket
+//                offsets, the task list (a task is <= rmax records of one
+//                bucket) and the list of buckets split over several tasks.
+//   k_scatter    forms per row S = sum_{f,p} fl32(weight*vis) (NaN real parts
+//                skipped, w-stacking.cu:517) and writes a record {pos_u,
+//                pos_v, S} to each of its buckets; slots come from one
+//                returning global atomic per distinct key per workgroup.
+//   k_zero_split clears the tiles of split buckets (overwrite mode).
+//   k_tile       one workgroup per task: accumulates its records into an LDS
+//                tile (one lane per footprint cell, LDS atomics), then writes
+//                the tile once with plain, coalesced stores; tasks of split
+//                buckets flush with HBM atomics instead.  Every grid cell is
+//                written by exactly one tile, so no memset of the grid is needed.
 //
-// The per-cell weight K[jKer]*K[kKer] is bitwise the reference's; the
-// per-cell value S*c differs from the reference's sum_{f,p} fl(fl32(w*v)*c)
-// only by rounding order (identical when F*P == 1).
+// The per-cell weight K[jKer]*K[kKer] is bitwise the reference's; the per-cell
+// value S*c differs from the reference's sum_{f,p} fl(fl32(w*v)*c) only by
+// rounding order (identical when F*P == 1).
 #include "rick_internal.h"
+
+#include <type_traits>
 
 #pragma clang fp contract(off)
 
 namespace rick {
 namespace {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 256;                 // count / scatter / scan workgroups
+constexpr int kRowsPerThread = 4;
+constexpr int kRowsPerBlock = kThreads * kRowsPerThread;
+constexpr int kHashBits = 12;
+constexpr int kHashSlots = 1 << kHashBits;    // >= distinct keys of one block (4 x 1024)
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr int kScanBlock = 4 * kThreads;      // buckets per scan block
+constexpr int kTileThreads = 512;
 constexpr int kPadU = 8;  // LDS row padding (elements): rows shift by 8 banks
 
 struct RecF64 {
@@ -91,30 +103,28 @@ __device__ __forceinline__ void uv_range(const GridGeom &g, double pu, double pv
     f.kmax = (f.gv < g.ny - g.kl) ? f.gv + g.kl : g.ny - 1;
 }
 
-__device__ __forceinline__ uint32_t bucket_id(const GridGeom &g, int w, int tv, int tu)
+// Buckets of a footprint (support <= tile size, so at most 2 tiles per axis).
+__device__ __forceinline__ int row_keys(const GridGeom &g, const Foot &f, uint32_t *keys)
 {
-    return ((uint32_t)w * (uint32_t)g.tiles_v + (uint32_t)tv) * (uint32_t)g.tiles_u + (uint32_t)tu;
+    const int tu0 = f.jmin / g.tu, tu1 = f.jmax / g.tu;
+    const int tv0 = f.kmin / g.tv, tv1 = f.kmax / g.tv;
+    const uint32_t base = (uint32_t)f.gw * (uint32_t)g.tiles_v;
+    int n = 0;
+    for (int tv = tv0; tv <= tv1; ++tv)
+        for (int tu = tu0; tu <= tu1; ++tu)
+            keys[n++] = (base + (uint32_t)tv) * (uint32_t)g.tiles_u + (uint32_t)tu;
+    return n;
 }
 
-// Wave-aggregated atomicAdd(&ctr[key], 1) for every active lane; returns the
-// counter value this lane obtained.  Must be called by the whole wave.
-__device__ __forceinline__ uint32_t agg_inc(uint32_t *ctr, uint32_t key, bool active)
+// LDS open-addressing hash set of bucket keys.
+__device__ __forceinline__ uint32_t hinsert(uint32_t *hkey, uint32_t key)
 {
-    const int lane = (int)__lane_id();
-    unsigned long long pending = __ballot(active);
-    uint32_t res = 0;
-    while (pending) {
-        const int leader = __ffsll(pending) - 1;
-        const uint32_t lkey = (uint32_t)__shfl((int)key, leader);
-        const unsigned long long peers = __ballot(active && key == lkey) & pending;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&ctr[lkey], (uint32_t)__popcll(peers));
-        base = (uint32_t)__shfl((int)base, leader);
-        if ((peers >> lane) & 1ull)
-            res = base + (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
-        pending &= ~peers;
+    uint32_t h = (key * 2654435761u) >> (32 - kHashBits);
+    while (true) {
+        const uint32_t old = atomicCAS(&hkey[h], kEmpty, key);
+        if (old == kEmpty || old == key) return h;
+        h = (h + 1) & (kHashSlots - 1);
     }
-    return res;
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v)
@@ -127,79 +137,161 @@ __global__ void __launch_bounds__(kThreads)
 k_count(GridGeom g, uint32_t n, const double *__restrict__ uu, const double *__restrict__ vv,
         const double *__restrict__ ww, uint32_t *__restrict__ counts)
 {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    Foot f{};
-    bool act = i < n;
-    if (act) act = footprint(g, uu[i] / g.dx, vv[i] / g.dx, ww[i] / g.dw, f);
-    const int tu0 = act ? f.jmin / g.tu : 0, tu1 = act ? f.jmax / g.tu : 0;
-    const int tv0 = act ? f.kmin / g.tv : 0, tv1 = act ? f.kmax / g.tv : 0;
+    __shared__ uint32_t hkey[kHashSlots], hcnt[kHashSlots];
+    for (int i = threadIdx.x; i < kHashSlots; i += kThreads) {
+        hkey[i] = kEmpty;
+        hcnt[i] = 0;
+    }
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kRowsPerBlock;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int du = q & 1, dv = q >> 1;
-        const bool a = act && tu0 + du <= tu1 && tv0 + dv <= tv1;
-        const uint32_t key = a ? bucket_id(g, f.gw, tv0 + dv, tu0 + du) : 0u;
-        agg_inc(counts, key, a);
+    for (int k = 0; k < kRowsPerThread; ++k) {
+        const uint32_t i = base + k * kThreads + threadIdx.x;
+        Foot f;
+        if (i < n && footprint(g, uu[i] / g.dx, vv[i] / g.dx, ww[i] / g.dw, f)) {
+            uint32_t keys[4];
+            const int nk = row_keys(g, f, keys);
+            for (int q = 0; q < nk; ++q) atomicAdd(&hcnt[hinsert(hkey, keys[q])], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHashSlots; i += kThreads) {
+        const uint32_t c = hcnt[i];
+        if (c) atomicAdd(&counts[hkey[i]], c);
     }
 }
 
-// Single-workgroup scan over nb buckets.
-constexpr int kScanThreads = 1024;
-
-__global__ void __launch_bounds__(kScanThreads)
-k_scan(GridGeom g, const uint32_t *__restrict__ counts, uint32_t *__restrict__ offsets,
-       uint32_t *__restrict__ cursor, uint32_t *__restrict__ task_off,
-       uint32_t *__restrict__ split_list, uint32_t split_cap, uint32_t *__restrict__ meta)
+// Scan of the histogram in three kernels: per-block sums, a single-block scan
+// of those, and the per-block downsweep that emits offsets and tasks.
+__device__ __forceinline__ uint32_t ntasks_of(uint32_t c, uint32_t rmax)
 {
-    __shared__ uint32_t s_rows[kScanThreads], s_tasks[kScanThreads], s_split[kScanThreads];
-    const uint32_t t = threadIdx.x;
-    const uint32_t nb = g.nb;
-    const uint32_t per = (nb + kScanThreads - 1) / kScanThreads;
-    const uint32_t b0 = t * per;
-    const uint32_t b1 = min(nb, b0 + per);
-    uint32_t rows = 0, tasks = 0, split = 0;
-    for (uint32_t b = b0; b < b1; ++b) {
-        const uint32_t c = counts[b];
-        const uint32_t nt = c == 0 ? 1u : (c + g.rmax - 1) / g.rmax;
-        rows += c;
-        tasks += nt;
-        split += nt > 1 ? 1u : 0u;
+    return c == 0 ? 1u : (c + rmax - 1) / rmax;
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_scan_reduce(GridGeom g, const uint32_t *__restrict__ counts, uint32_t *__restrict__ blocksum,
+              uint32_t nscan)
+{
+    __shared__ uint32_t s[3][kThreads];
+    uint32_t r = 0, t = 0, sp = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t b = blockIdx.x * kScanBlock + threadIdx.x * 4 + k;
+        if (b < g.nb) {
+            const uint32_t c = counts[b], nt = ntasks_of(c, g.rmax);
+            r += c;
+            t += nt;
+            sp += nt > 1 ? 1u : 0u;
+        }
     }
-    s_rows[t] = rows;
-    s_tasks[t] = tasks;
-    s_split[t] = split;
+    s[0][threadIdx.x] = r;
+    s[1][threadIdx.x] = t;
+    s[2][threadIdx.x] = sp;
     __syncthreads();
-    // Hillis-Steele inclusive scan.
-    for (uint32_t d = 1; d < kScanThreads; d <<= 1) {
-        uint32_t a = 0, b = 0, c = 0;
-        if (t >= d) {
-            a = s_rows[t - d];
-            b = s_tasks[t - d];
-            c = s_split[t - d];
+    for (int d = kThreads / 2; d > 0; d >>= 1) {
+        if ((int)threadIdx.x < d) {
+            s[0][threadIdx.x] += s[0][threadIdx.x + d];
+            s[1][threadIdx.x] += s[1][threadIdx.x + d];
+            s[2][threadIdx.x] += s[2][threadIdx.x + d];
         }
         __syncthreads();
-        s_rows[t] += a;
-        s_tasks[t] += b;
-        s_split[t] += c;
+    }
+    if (threadIdx.x == 0) {
+        blocksum[blockIdx.x] = s[0][0];
+        blocksum[nscan + blockIdx.x] = s[1][0];
+        blocksum[2 * nscan + blockIdx.x] = s[2][0];
+    }
+}
+
+constexpr int kTopThreads = 1024;
+
+__global__ void __launch_bounds__(kTopThreads)
+k_scan_top(GridGeom g, uint32_t *__restrict__ blocksum, uint32_t nscan,
+           uint32_t *__restrict__ offsets, uint32_t *__restrict__ meta)
+{
+    __shared__ uint32_t s[3][kTopThreads];
+    uint32_t carry[3] = {0, 0, 0};
+    for (uint32_t c0 = 0; c0 < nscan; c0 += kTopThreads) {
+        const uint32_t i = c0 + threadIdx.x;
+        uint32_t v[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            v[a] = i < nscan ? blocksum[a * nscan + i] : 0u;
+            s[a][threadIdx.x] = v[a];
+        }
+        __syncthreads();
+        for (int d = 1; d < kTopThreads; d <<= 1) {
+            uint32_t add[3] = {0, 0, 0};
+            if ((int)threadIdx.x >= d)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) add[a] = s[a][threadIdx.x - d];
+            __syncthreads();
+#pragma unroll
+            for (int a = 0; a < 3; ++a) s[a][threadIdx.x] += add[a];
+            __syncthreads();
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (i < nscan) blocksum[a * nscan + i] = carry[a] + s[a][threadIdx.x] - v[a];
+            carry[a] += s[a][kTopThreads - 1];
+        }
         __syncthreads();
     }
-    uint32_t ro = s_rows[t] - rows, to = s_tasks[t] - tasks, so = s_split[t] - split;
-    for (uint32_t b = b0; b < b1; ++b) {
-        const uint32_t c = counts[b];
-        const uint32_t nt = c == 0 ? 1u : (c + g.rmax - 1) / g.rmax;
+    if (threadIdx.x == 0) {
+        offsets[g.nb] = carry[0];
+        meta[0] = carry[0];
+        meta[1] = carry[1];
+        meta[2] = carry[2];
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_scan_down(GridGeom g, const uint32_t *__restrict__ counts, const uint32_t *__restrict__ blockoff,
+            uint32_t nscan, uint32_t *__restrict__ offsets, uint32_t *__restrict__ cursor,
+            uint2 *__restrict__ tasks, uint32_t task_cap, uint32_t *__restrict__ split_list,
+            uint32_t split_cap)
+{
+    __shared__ uint32_t s[3][kThreads];
+    uint32_t c[4], v[3] = {0, 0, 0};
+    const uint32_t b0 = blockIdx.x * kScanBlock + threadIdx.x * 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        c[k] = b0 + k < g.nb ? counts[b0 + k] : 0u;
+        const uint32_t nt = ntasks_of(c[k], g.rmax);
+        if (b0 + k < g.nb) {
+            v[0] += c[k];
+            v[1] += nt;
+            v[2] += nt > 1 ? 1u : 0u;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) s[a][threadIdx.x] = v[a];
+    __syncthreads();
+    for (int d = 1; d < kThreads; d <<= 1) {
+        uint32_t add[3] = {0, 0, 0};
+        if ((int)threadIdx.x >= d)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) add[a] = s[a][threadIdx.x - d];
+        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < 3; ++a) s[a][threadIdx.x] += add[a];
+        __syncthreads();
+    }
+    uint32_t ro = blockoff[blockIdx.x] + s[0][threadIdx.x] - v[0];
+    uint32_t to = blockoff[nscan + blockIdx.x] + s[1][threadIdx.x] - v[1];
+    uint32_t so = blockoff[2 * nscan + blockIdx.x] + s[2][threadIdx.x] - v[2];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t b = b0 + k;
+        if (b >= g.nb) break;
         offsets[b] = ro;
         cursor[b] = ro;
-        task_off[b] = to;
-        if (nt > 1 && so < split_cap) split_list[so] = b;
-        ro += c;
+        const uint32_t nt = ntasks_of(c[k], g.rmax);
+        for (uint32_t ch = 0; ch < nt; ++ch)
+            if (to + ch < task_cap) tasks[to + ch] = make_uint2(b, ch);
+        if (nt > 1 && so < split_cap) split_list[so++] = b;
+        ro += c[k];
         to += nt;
-        so += nt > 1 ? 1u : 0u;
-    }
-    if (t == kScanThreads - 1) {
-        offsets[nb] = s_rows[t];
-        task_off[nb] = s_tasks[t];
-        meta[0] = s_rows[t];
-        meta[1] = s_tasks[t];
-        meta[2] = s_split[t];
     }
 }
 
@@ -211,54 +303,82 @@ k_scatter(GridGeom g, uint32_t n, uint32_t F, uint32_t P, const double *__restri
           const float *__restrict__ wt, uint32_t *__restrict__ cursor,
           typename RecOf<PREC>::type *__restrict__ rec)
 {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    Foot f{};
-    double pu = 0.0, pv = 0.0;
-    bool act = i < n;
-    if (act) {
-        pu = uu[i] / g.dx;
-        pv = vv[i] / g.dx;
-        act = footprint(g, pu, pv, ww[i] / g.dw, f);
+    __shared__ uint32_t hkey[kHashSlots], hcnt[kHashSlots], hbase[kHashSlots];
+    for (int i = threadIdx.x; i < kHashSlots; i += kThreads) {
+        hkey[i] = kEmpty;
+        hcnt[i] = 0;
     }
-    // Channel/polarization sum, w-stacking.cu:512-527 (weights indexed
-    // row*P + p, shared across channels; float product first).
-    double sre = 0.0, sim = 0.0;
-    if (act) {
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kRowsPerBlock;
+    double pu[kRowsPerThread], pv[kRowsPerThread], sre[kRowsPerThread], sim[kRowsPerThread];
+    uint32_t item[kRowsPerThread][4];
+    int nk[kRowsPerThread];
+#pragma unroll
+    for (int k = 0; k < kRowsPerThread; ++k) {
+        const uint32_t i = base + k * kThreads + threadIdx.x;
+        nk[k] = 0;
+        pu[k] = pv[k] = sre[k] = sim[k] = 0.0;
+        Foot f;
+        if (i >= n) continue;
+        pu[k] = uu[i] / g.dx;
+        pv[k] = vv[i] / g.dx;
+        if (!footprint(g, pu[k], pv[k], ww[i] / g.dw, f)) continue;
+        // Channel/polarization sum, w-stacking.cu:512-527 (weights indexed
+        // row*P + p, shared across channels; float product first).
         const size_t vb = (size_t)i * F * P;
         const float *wrow = wt + (size_t)i * P;
+        double a_re = 0.0, a_im = 0.0;
         for (uint32_t fr = 0; fr < F; ++fr) {
             for (uint32_t p = 0; p < P; ++p) {
-                const size_t k = vb + (size_t)fr * P + p;
-                const float a = vr[k];
+                const size_t q = vb + (size_t)fr * P + p;
+                const float a = vr[q];
                 if (!isnan(a)) {
                     const float wr = wrow[p] * a;
-                    const float wi = wrow[p] * vi[k];
-                    sre += (double)wr;
-                    sim += (double)wi;
+                    const float wi = wrow[p] * vi[q];
+                    a_re += (double)wr;
+                    a_im += (double)wi;
                 }
             }
         }
-    }
-    const int tu0 = act ? f.jmin / g.tu : 0, tu1 = act ? f.jmax / g.tu : 0;
-    const int tv0 = act ? f.kmin / g.tv : 0, tv1 = act ? f.kmax / g.tv : 0;
+        sre[k] = a_re;
+        sim[k] = a_im;
+        uint32_t keys[4];
+        nk[k] = row_keys(g, f, keys);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int du = q & 1, dv = q >> 1;
-        const bool a = act && tu0 + du <= tu1 && tv0 + dv <= tv1;
-        const uint32_t key = a ? bucket_id(g, f.gw, tv0 + dv, tu0 + du) : 0u;
-        const uint32_t slot = agg_inc(cursor, key, a);
-        if (a) {
-            typename RecOf<PREC>::type r;
-            r.pu = pu;
-            r.pv = pv;
-            if constexpr (PREC == RICK_FP64) {
-                r.sre = sre;
-                r.sim = sim;
-            } else {
-                r.sre = (float)sre;
-                r.sim = (float)sim;
+        for (int q = 0; q < 4; ++q) {
+            if (q < nk[k]) {
+                const uint32_t slot = hinsert(hkey, keys[q]);
+                const uint32_t rank = atomicAdd(&hcnt[slot], 1u);
+                item[k][q] = slot | (rank << kHashBits);
             }
-            rec[slot] = r;
+        }
+    }
+    __syncthreads();
+    // One returning global atomic per distinct key, all issued together.
+    for (int i = threadIdx.x; i < kHashSlots; i += kThreads) {
+        const uint32_t c = hcnt[i];
+        if (c) hbase[i] = atomicAdd(&cursor[hkey[i]], c);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kRowsPerThread; ++k) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (q < nk[k]) {
+                const uint32_t it = item[k][q];
+                const uint32_t pos = hbase[it & (kHashSlots - 1)] + (it >> kHashBits);
+                typename RecOf<PREC>::type r;
+                r.pu = pu[k];
+                r.pv = pv[k];
+                if constexpr (PREC == RICK_FP64) {
+                    r.sre = sre[k];
+                    r.sim = sim[k];
+                } else {
+                    r.sre = (float)sre[k];
+                    r.sim = (float)sim[k];
+                }
+                rec[pos] = r;
+            }
         }
     }
 }
@@ -301,42 +421,53 @@ __device__ __forceinline__ float readlane_f32(float x, int l)
 // atomics with an exactly captured rounding error (TwoSum on the value the
 // returning atomic saw) accumulated in a second FP32 array.
 template <int PREC, int TU, int TV>
-__global__ void __launch_bounds__(kThreads)
-k_tile(GridGeom g, const uint32_t *__restrict__ offsets, const uint32_t *__restrict__ task_off,
+__global__ void __launch_bounds__(kTileThreads)
+k_tile(GridGeom g, const uint32_t *__restrict__ offsets, const uint2 *__restrict__ tasks,
        const uint32_t *__restrict__ meta, const typename RecOf<PREC>::type *__restrict__ rec,
        const double *__restrict__ table, void *grid_v, int accumulate)
 {
     using T = typename std::conditional<PREC == RICK_FP64, double, float>::type;
+    using T2 = typename std::conditional<PREC == RICK_FP64, double2, float2>::type;
     constexpr int LDU = TU + kPadU;
     constexpr int NARR = PREC == RICK_FP64 ? 2 : 4;
+    constexpr int NWAVES = kTileThreads / 64;
     __shared__ T lds[NARR * TV * LDU];
     __shared__ T stab[kMaxTable];
     T *sre = lds, *sim = lds + TV * LDU;
     T *cre = lds + 2 * TV * LDU, *cim = lds + 3 * TV * LDU;  // FP32 only
 
-    const uint32_t task = blockIdx.x;
-    if (task >= meta[1]) return;
-    // Bucket of this task: largest b with task_off[b] <= task (every bucket
-    // has at least one task, so task_off is strictly increasing).
-    uint32_t lo = 0, hi = g.nb;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (task_off[mid] <= task) lo = mid; else hi = mid;
-    }
-    const uint32_t b = lo;
-    const uint32_t chunk = task - task_off[b];
-    const bool split = task_off[b + 1] - task_off[b] > 1;
-    const uint32_t r0 = offsets[b] + chunk * g.rmax;
-    const uint32_t r1 = min(r0 + g.rmax, offsets[b + 1]);
+    if (blockIdx.x >= meta[1]) return;
+    const uint2 task = tasks[blockIdx.x];
+    const uint32_t b = task.x;
+    const uint32_t beg = offsets[b], end = offsets[b + 1];
+    const bool split = end - beg > g.rmax;
+    const uint32_t r0 = beg + task.y * g.rmax;
+    const uint32_t r1 = min(r0 + g.rmax, end);
 
     const uint32_t per_plane = (uint32_t)g.tiles_u * (uint32_t)g.tiles_v;
     const int w = (int)(b / per_plane), rem = (int)(b % per_plane);
     const int tv = rem / g.tiles_u, tu = rem % g.tiles_u;
     const int j0 = tu * TU, k0 = tv * TV;
     const int nj = min(TU, g.nx - j0), nk = min(TV, g.ny - k0);
+    T *grid = static_cast<T *>(grid_v);
 
-    for (int idx = threadIdx.x; idx < NARR * TV * LDU; idx += kThreads) lds[idx] = T(0);
-    for (int idx = threadIdx.x; idx < g.ntable; idx += kThreads) stab[idx] = (T)table[idx];
+    if (r0 >= r1) {
+        // Empty bucket: its cells are zero (overwrite) or unchanged (accumulate).
+        if (accumulate) return;
+        for (int idx = threadIdx.x; idx < nk * TU; idx += kTileThreads) {
+            const int kk = idx / TU, jj = idx % TU;
+            if (jj >= nj) continue;
+            const size_t c = ((size_t)w * g.ny + (size_t)(k0 + kk)) * (size_t)g.nx + (size_t)(j0 + jj);
+            T2 z;
+            z.x = T(0);
+            z.y = T(0);
+            reinterpret_cast<T2 *>(grid)[c] = z;
+        }
+        return;
+    }
+
+    for (int idx = threadIdx.x; idx < NARR * TV * LDU; idx += kTileThreads) lds[idx] = T(0);
+    for (int idx = threadIdx.x; idx < g.ntable; idx += kTileThreads) stab[idx] = (T)table[idx];
     __syncthreads();
 
     const int lane = (int)(threadIdx.x & 63);
@@ -346,7 +477,7 @@ k_tile(GridGeom g, const uint32_t *__restrict__ offsets, const uint32_t *__restr
     const double std22 = 1.0 / (2.0 * 1.0 * 1.0);
     const double gnorm = std22 / kRefPi;
 
-    for (uint32_t base = r0 + (uint32_t)wave * 64u; base < r1; base += 4u * 64u) {
+    for (uint32_t base = r0 + (uint32_t)wave * 64u; base < r1; base += NWAVES * 64u) {
         // Each lane fetches one record; rows are then broadcast one by one.
         const uint32_t my = base + (uint32_t)lane;
         double mpu = 0.0, mpv = 0.0;
@@ -411,8 +542,7 @@ k_tile(GridGeom g, const uint32_t *__restrict__ offsets, const uint32_t *__restr
     }
     __syncthreads();
 
-    T *grid = static_cast<T *>(grid_v);
-    for (int idx = threadIdx.x; idx < nk * TU; idx += kThreads) {
+    for (int idx = threadIdx.x; idx < nk * TU; idx += kTileThreads) {
         const int kk = idx / TU, jj = idx % TU;
         if (jj >= nj) continue;
         const int li = kk * LDU + jj;
@@ -426,7 +556,6 @@ k_tile(GridGeom g, const uint32_t *__restrict__ offsets, const uint32_t *__restr
             if (vre != T(0)) unsafeAtomicAdd(&grid[2 * c], vre);
             if (vim != T(0)) unsafeAtomicAdd(&grid[2 * c + 1], vim);
         } else {
-            using T2 = typename std::conditional<PREC == RICK_FP64, double2, float2>::type;
             T2 *g2 = reinterpret_cast<T2 *>(grid) + c;
             T2 v;
             v.x = vre;
@@ -447,14 +576,14 @@ hipError_t launch_tile(const GridGeom &g, const GridWork &w, const double *table
 {
     using Rec = typename RecOf<PREC>::type;
     const Rec *rec = static_cast<const Rec *>(w.records);
-    const dim3 grd(w.task_cap), blk(kThreads);
-#define RICK_TILE_CASE(A, B)                                                                 \
-    if (g.tu == A && g.tv == B) {                                                            \
-        tm->begin(RICK_STAGE_TILE, s);                                                       \
-        hipLaunchKernelGGL((k_tile<PREC, A, B>), grd, blk, 0, s, g, w.offsets, w.task_off,  \
-                           w.meta, rec, table, grid, accumulate);                            \
-        tm->end(RICK_STAGE_TILE, s);                                                         \
-        return hipGetLastError();                                                            \
+    const dim3 grd(w.task_cap), blk(kTileThreads);
+#define RICK_TILE_CASE(A, B)                                                               \
+    if (g.tu == A && g.tv == B) {                                                          \
+        tm->begin(RICK_STAGE_TILE, s);                                                     \
+        hipLaunchKernelGGL((k_tile<PREC, A, B>), grd, blk, 0, s, g, w.offsets, w.tasks,   \
+                           w.meta, rec, table, grid, accumulate);                          \
+        tm->end(RICK_STAGE_TILE, s);                                                       \
+        return hipGetLastError();                                                          \
     }
     RICK_TILE_CASE(64, 64)
     RICK_TILE_CASE(64, 32)
@@ -467,31 +596,33 @@ hipError_t launch_tile(const GridGeom &g, const GridWork &w, const double *table
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-}  // namespace
-
 // Capacities: a footprint (support <= tile size) touches at most 2 tiles per
 // axis, so a row lands in at most 4 buckets; a split bucket holds more than
-// rmax rows, so there are at most 4*rows/(rmax+1) of them; tasks number at
+// rmax records, so there are at most 4*rows/(rmax+1) of them; tasks number at
 // most nb + 4*rows/rmax.
-static void capacities(const GridGeom &g, uint32_t max_points, uint32_t *rec_cap,
-                       uint32_t *split_cap, uint32_t *task_cap)
+void capacities(const GridGeom &g, uint32_t max_points, uint32_t *rec_cap, uint32_t *split_cap,
+                uint32_t *task_cap, uint32_t *nscan)
 {
     const uint64_t recs = 4ull * (uint64_t)max_points;
     *rec_cap = (uint32_t)(recs > 0 ? recs : 1);
     *split_cap = (uint32_t)(recs / ((uint64_t)g.rmax + 1) + 1);
     *task_cap = (uint32_t)((uint64_t)g.nb + recs / g.rmax + 1);
+    *nscan = (g.nb + kScanBlock - 1) / kScanBlock;
 }
+
+}  // namespace
 
 size_t grid_work_bytes(const GridGeom &g, uint32_t max_points, int precision)
 {
-    uint32_t rec_cap, split_cap, task_cap;
-    capacities(g, max_points, &rec_cap, &split_cap, &task_cap);
+    uint32_t rec_cap, split_cap, task_cap, nscan;
+    capacities(g, max_points, &rec_cap, &split_cap, &task_cap, &nscan);
     const size_t rs = precision == RICK_FP64 ? sizeof(RecF64) : sizeof(RecF32);
     size_t b = 0;
     b += align_up(sizeof(uint32_t) * g.nb, 256);           // counts
     b += align_up(sizeof(uint32_t) * g.nb, 256);           // cursor
     b += align_up(sizeof(uint32_t) * (g.nb + 1), 256);     // offsets
-    b += align_up(sizeof(uint32_t) * (g.nb + 1), 256);     // task_off
+    b += align_up(sizeof(uint32_t) * 3 * nscan, 256);      // blocksum
+    b += align_up(sizeof(uint2) * task_cap, 256);          // tasks
     b += align_up(sizeof(uint32_t) * split_cap, 256);      // split_list
     b += align_up(sizeof(uint32_t) * 4, 256);              // meta
     b += align_up(rs * rec_cap, 256);                      // records
@@ -501,8 +632,8 @@ size_t grid_work_bytes(const GridGeom &g, uint32_t max_points, int precision)
 void grid_work_carve(const GridGeom &g, uint32_t max_points, int precision, void *base,
                      GridWork *w)
 {
-    uint32_t rec_cap, split_cap, task_cap;
-    capacities(g, max_points, &rec_cap, &split_cap, &task_cap);
+    uint32_t rec_cap, split_cap, task_cap, nscan;
+    capacities(g, max_points, &rec_cap, &split_cap, &task_cap, &nscan);
     const size_t rs = precision == RICK_FP64 ? sizeof(RecF64) : sizeof(RecF32);
     char *p = static_cast<char *>(base);
     auto take = [&](size_t bytes) {
@@ -513,13 +644,15 @@ void grid_work_carve(const GridGeom &g, uint32_t max_points, int precision, void
     w->counts = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * g.nb));
     w->cursor = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * g.nb));
     w->offsets = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * (g.nb + 1)));
-    w->task_off = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * (g.nb + 1)));
+    w->blocksum = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * 3 * nscan));
+    w->tasks = reinterpret_cast<uint2 *>(take(sizeof(uint2) * task_cap));
     w->split_list = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * split_cap));
     w->meta = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * 4));
     w->records = take(rs * rec_cap);
     w->rec_cap = rec_cap;
     w->split_cap = split_cap;
     w->task_cap = task_cap;
+    w->nscan = nscan;
 }
 
 hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
@@ -530,7 +663,7 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
 {
     Timing off;
     if (tm == nullptr) tm = &off;
-    const uint32_t nblk = (n + kThreads - 1) / kThreads;
+    const uint32_t nblk = (n + kRowsPerBlock - 1) / kRowsPerBlock;
     const uint32_t fill_blocks = min((g.nb + 255u) / 256u, 4096u);
     tm->begin(RICK_STAGE_COUNT_ROWS, s);
     hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks), dim3(256), 0, s, w.counts, g.nb, 0u);
@@ -538,33 +671,38 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
         hipLaunchKernelGGL(k_count, dim3(nblk), dim3(kThreads), 0, s, g, n, uu, vv, ww, w.counts);
     tm->end(RICK_STAGE_COUNT_ROWS, s);
     tm->begin(RICK_STAGE_SCAN, s);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, s, g, w.counts, w.offsets,
-                       w.cursor, w.task_off, w.split_list, w.split_cap, w.meta);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(w.nscan), dim3(kThreads), 0, s, g, w.counts,
+                       w.blocksum, w.nscan);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kTopThreads), 0, s, g, w.blocksum, w.nscan,
+                       w.offsets, w.meta);
+    hipLaunchKernelGGL(k_scan_down, dim3(w.nscan), dim3(kThreads), 0, s, g, w.counts, w.blocksum,
+                       w.nscan, w.offsets, w.cursor, w.tasks, w.task_cap, w.split_list,
+                       w.split_cap);
     tm->end(RICK_STAGE_SCAN, s);
-    if (precision == RICK_FP64) {
-        tm->begin(RICK_STAGE_SCATTER, s);
-        if (n > 0)
+    tm->begin(RICK_STAGE_SCATTER, s);
+    if (n > 0) {
+        if (precision == RICK_FP64)
             hipLaunchKernelGGL(k_scatter<RICK_FP64>, dim3(nblk), dim3(kThreads), 0, s, g, n, F, P,
                                uu, vv, ww, vr, vi, wt, w.cursor,
                                static_cast<RecF64 *>(w.records));
-        tm->end(RICK_STAGE_SCATTER, s);
-        tm->begin(RICK_STAGE_ZERO, s);
-        if (!accumulate)
-            hipLaunchKernelGGL(k_zero_split<double>, dim3(w.split_cap), dim3(kThreads), 0, s, g,
-                               w.split_list, w.meta, static_cast<double *>(grid));
-        tm->end(RICK_STAGE_ZERO, s);
-        return launch_tile<RICK_FP64>(g, w, table_d, grid, accumulate, s, tm);
+        else
+            hipLaunchKernelGGL(k_scatter<RICK_FP32>, dim3(nblk), dim3(kThreads), 0, s, g, n, F, P,
+                               uu, vv, ww, vr, vi, wt, w.cursor,
+                               static_cast<RecF32 *>(w.records));
     }
-    tm->begin(RICK_STAGE_SCATTER, s);
-    if (n > 0)
-        hipLaunchKernelGGL(k_scatter<RICK_FP32>, dim3(nblk), dim3(kThreads), 0, s, g, n, F, P, uu,
-                           vv, ww, vr, vi, wt, w.cursor, static_cast<RecF32 *>(w.records));
     tm->end(RICK_STAGE_SCATTER, s);
     tm->begin(RICK_STAGE_ZERO, s);
-    if (!accumulate)
-        hipLaunchKernelGGL(k_zero_split<float>, dim3(w.split_cap), dim3(kThreads), 0, s, g,
-                           w.split_list, w.meta, static_cast<float *>(grid));
+    if (!accumulate) {
+        if (precision == RICK_FP64)
+            hipLaunchKernelGGL(k_zero_split<double>, dim3(w.split_cap), dim3(kThreads), 0, s, g,
+                               w.split_list, w.meta, static_cast<double *>(grid));
+        else
+            hipLaunchKernelGGL(k_zero_split<float>, dim3(w.split_cap), dim3(kThreads), 0, s, g,
+                               w.split_list, w.meta, static_cast<float *>(grid));
+    }
     tm->end(RICK_STAGE_ZERO, s);
+    if (precision == RICK_FP64)
+        return launch_tile<RICK_FP64>(g, w, table_d, grid, accumulate, s, tm);
     return launch_tile<RICK_FP32>(g, w, table_d, grid, accumulate, s, tm);
 }
 

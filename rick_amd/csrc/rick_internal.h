@@ -59,14 +59,15 @@ struct GridGeom {
 
 // Workspace of the binned gridder, carved out of one device allocation.
 struct GridWork {
-    uint32_t *counts;       // [nb]  rows per bucket
+    uint32_t *counts;       // [nb]  records per bucket
     uint32_t *cursor;       // [nb]  scatter cursor (starts at the bucket offset)
     uint32_t *offsets;      // [nb+1] exclusive scan of counts
-    uint32_t *task_off;     // [nb+1] exclusive scan of tasks per bucket
+    uint32_t *blocksum;     // [3*nscan] per scan block: records, tasks, split buckets
+    uint2 *tasks;           // [task_cap] (bucket, chunk) per gridding task
     uint32_t *split_list;   // [split_cap] buckets handled by more than one task
-    uint32_t *meta;         // [4]: total bucket rows, tasks, split buckets
+    uint32_t *meta;         // [4]: total records, tasks, split buckets
     void *records;          // [rec_cap] sorted row records
-    uint32_t rec_cap, split_cap, task_cap;
+    uint32_t rec_cap, split_cap, task_cap, nscan;
 };
 
 size_t grid_work_bytes(const GridGeom &g, uint32_t max_points, int precision);
