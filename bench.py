@@ -46,6 +46,13 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32"], default=None,
                     help="run one path only (for profiler runs)")
+    ap.add_argument("--tile", default=None, help="gridder tile TUxTV (default: tuned/64x64)")
+    ap.add_argument("--rows-per-task", type=int, default=0)
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune tile shape and rows/task first (rick_amd.autotune)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run one 1 GiB device-to-device copy (known HBM bytes) so that "
+                         "profiler byte counters can be calibrated against it")
     return ap.parse_args()
 
 
@@ -62,11 +69,11 @@ def timed(fn, steps, torch):
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
-def run_path(ds, ins, precision, steps, warmup, torch):
+def run_path(ds, ins, precision, steps, warmup, torch, tile=(0, 0), rpt=0):
     from rick_amd import imaging
     cfg = ds.cfg
     F, P = cfg.n_chan, cfg.n_pol
-    plan = imaging.plan_for(ds, precision=precision)
+    plan = imaging.plan_for(ds, precision=precision, tile=tile, rows_per_task=rpt)
     grid = plan.new_grid()
     npix = cfg.grid * cfg.grid
     re = torch.empty(npix, dtype=torch.float64, device="cuda:0")
@@ -150,9 +157,16 @@ def main():
         sys.exit(2)
     import torch
 
-    from rick_amd import FP32, FP64, synth
+    from rick_amd import FP32, FP64, autotune, synth
 
     cfg = synth.CONFIGS[args.config]
+    if args.copy_probe:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: 1 GiB read + 1 GiB written by one copy kernel")
+        del src, dst
     log(f"generating {cfg.name}: {cfg.n_rows} rows x {cfg.n_chan} ch = {cfg.n_vis} vis")
     ds = synth.make_dataset(cfg, device="cuda:0")
     ins = [ds.uu, ds.vv, ds.ww, ds.vis_real, ds.vis_img, ds.weight]
@@ -162,12 +176,24 @@ def main():
     for name, prec in (("fp64", FP64), ("fp32", FP32)):
         if args.only and args.only != name:
             continue
-        log(f"{name}: warmup {args.warmup}, steps {args.steps}")
+        tile, rpt = (0, 0), args.rows_per_task
+        if args.tile:
+            tile = tuple(int(x) for x in args.tile.lower().split("x"))
+        elif args.tune:
+            best = autotune.tune_grid(ds, prec, log=log)["best"]
+            tile, rpt = tuple(best["tile"]), best["rows_per_task"]
+        else:
+            tuned = autotune.lookup(cfg.grid, cfg.planes, cfg.support, prec, cfg.n_rows)
+            if tuned:
+                tile, rpt = tuned
+        log(f"{name}: tile {tile} rows/task {rpt}; warmup {args.warmup}, steps {args.steps}")
         plan, gms, ims, stages, stats, img = run_path(ds, ins, prec, args.steps, args.warmup,
-                                                      torch)
+                                                      torch, tile, rpt)
         res[name] = {"grid_ms": gms, "image_ms": ims, "grid_vis_per_s": vis / (gms * 1e-3),
                      "stages": stages, "grid_stats": stats, "img": img,
-                     "grid_bytes": plan.grid_bytes}
+                     "grid_bytes": plan.grid_bytes,
+                     "tile": [plan.desc.tile_u or 64, plan.desc.tile_v or 64],
+                     "rows_per_task": plan.desc.rows_per_task or 4096}
         log(f"{name}: grid {gms:.3f} ms ({vis / gms / 1e6:.3f} Gvis/s), image {ims:.3f} ms")
         del plan
         torch.cuda.empty_cache()
@@ -178,10 +204,13 @@ def main():
     cb = 8 if main_path == "fp32" else 16
     # Dominant gridding kernel and its algorithmic bytes.
     grid_stages = {k: v for k, v in r["stages"].items()
-                   if k in ("count", "scan", "scatter", "zero_split", "tile")}
+                   if k in ("count", "scan", "scatter", "zero_empty", "zero_split", "tile")}
     dom = max(grid_stages, key=lambda k: grid_stages[k]["ms_per_step"])
     grid_bytes = cfg.planes * cfg.grid * cfg.grid * cb
-    alg = {"tile": grid_bytes,                              # every grid cell written once
+    tile_bytes = r["tile"][0] * r["tile"][1] * cb
+    nonempty = r["grid_stats"]["nonempty_buckets"]
+    alg = {"tile": nonempty * tile_bytes,                   # cells of non-empty tiles, once
+           "zero_empty": grid_bytes - nonempty * tile_bytes,  # cells of empty tiles, once
            "scatter": rows * (24 + 4 * P) + vis * 8,        # uvw, weights, visibilities read once
            "count": rows * 24}.get(dom)
     t_dom = grid_stages[dom]["ms_per_launch"] * 1e-3
@@ -220,6 +249,7 @@ def main():
     for name, v in res.items():
         out[name] = {"grid_ms": v["grid_ms"], "image_ms": v["image_ms"],
                      "grid_vis_per_s": v["grid_vis_per_s"], "grid_stats": v["grid_stats"],
+                     "tile": v["tile"], "rows_per_task": v["rows_per_task"],
                      "stages_ms_per_step": {k: round(s["ms_per_step"], 4)
                                             for k, s in v["stages"].items()}}
     if "fp32" in res and "fp64" in res:

@@ -151,13 +151,15 @@ enum {
     RICK_STAGE_COUNT_ROWS = 0, /* bucket histogram (incl. clearing it)        */
     RICK_STAGE_SCAN = 1,       /* bucket / task offsets                        */
     RICK_STAGE_SCATTER = 2,    /* per-row channel sums -> sorted records       */
-    RICK_STAGE_ZERO = 3,       /* clear tiles of split buckets                 */
+    RICK_STAGE_ZERO = 3,       /* clear tiles of empty buckets (side stream,   */
+                               /* concurrent with stages 1, 2, 4, 9)          */
     RICK_STAGE_TILE = 4,       /* LDS-tile accumulation and grid write         */
     RICK_STAGE_FFT_COLS = 5,   /* column FFT (single pass, or four-step A)     */
     RICK_STAGE_FFT_COLS_B = 6, /* column FFT four-step B                       */
     RICK_STAGE_FFT_ROWS = 7,   /* row FFT (+ 1/(Nx*Ny))                         */
     RICK_STAGE_PHASE = 8,      /* phase correction + plane sum                 */
-    RICK_STAGE_COUNT = 9
+    RICK_STAGE_ZERO_SPLIT = 9, /* clear tiles of split buckets                 */
+    RICK_STAGE_COUNT = 10
 };
 int rick_plan_profile(rick_plan plan, int enable);
 int rick_plan_stage_times(rick_plan plan, double *ms, int *launches, int n);
@@ -165,7 +167,7 @@ int rick_plan_stage_times(rick_plan plan, double *ms, int *launches, int n);
 /* Binning statistics of the last rick_plan_grid call (waits for it):
  * stats[0] = row records written (rows x tiles each row touches),
  * stats[1] = gridding tasks (workgroups), stats[2] = buckets split over
- * several tasks.  n <= 3 entries are written. */
+ * several tasks, stats[3] = non-empty buckets.  n <= 4 entries are written. */
 int rick_plan_grid_stats(rick_plan plan, unsigned int *stats, int n);
 
 /* Whole chain: grid (overwrite) -> FFT in place -> phase into the images

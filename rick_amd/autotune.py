@@ -1,0 +1,100 @@
+"""Autotuner for the gridder's launch shape.
+
+The gridder's tile shape (TU x TV cells per LDS tile, i.e. per bucket) and the
+records per task (rows_per_task) trade occupancy (LDS per workgroup), record
+duplication across tile borders and load balance on hot tiles.  This module
+times the candidates on the actual workload and caches the fastest per
+workload signature in a JSON file (rick_amd/tuning.json by default).
+
+    python -m rick_amd.autotune --config lofar --precision fp32
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import time
+from pathlib import Path
+
+TILES = [(64, 64), (64, 32), (32, 32), (128, 32), (32, 64)]
+ROWS_PER_TASK = [1024, 4096, 16384]
+CACHE = Path(__file__).resolve().parent / "tuning.json"
+
+
+def signature(grid: int, planes: int, support: int, precision: int, rows: int) -> str:
+    """Tuning key: grid, planes, support, precision and log2-bucketed row count."""
+    return f"g{grid}_p{planes}_s{support}_{'fp32' if precision else 'fp64'}_r{rows.bit_length()}"
+
+
+def load(path: Path = CACHE) -> dict:
+    try:
+        return json.loads(Path(path).read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def lookup(grid, planes, support, precision, rows, path: Path = CACHE):
+    """(tile, rows_per_task) tuned for this shape, or None."""
+    e = load(path).get(signature(grid, planes, support, precision, rows))
+    if not e:
+        return None
+    return tuple(e["tile"]), int(e["rows_per_task"])
+
+
+def tune_grid(ds, precision, tiles=None, rpts=None, steps=3, log=print):
+    """Time rick_plan_grid for each candidate on dataset ``ds`` (inputs on the
+    GPU); returns {"best": {...}, "results": [...]}."""
+    import torch
+
+    from .imaging import plan_for
+    cfg = ds.cfg
+    ins = [ds.uu, ds.vv, ds.ww, ds.vis_real, ds.vis_img, ds.weight]
+    results = []
+    for tile in tiles or TILES:
+        if tile[0] < cfg.support or tile[1] < cfg.support:
+            continue
+        for rpt in rpts or ROWS_PER_TASK:
+            plan = plan_for(ds, precision=precision, tile=tile, rows_per_task=rpt)
+            grid = plan.new_grid()
+            plan.grid(*ins, cfg.n_chan, cfg.n_pol, out=grid)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                plan.grid(*ins, cfg.n_chan, cfg.n_pol, out=grid)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / steps
+            results.append({"tile": list(tile), "rows_per_task": rpt, "ms": ms})
+            log(f"[tune] tile {tile[0]}x{tile[1]} rows/task {rpt}: {ms:.3f} ms")
+            del grid, plan
+            torch.cuda.empty_cache()
+    best = min(results, key=lambda r: r["ms"])
+    return {"best": best, "results": results}
+
+
+def store(key: str, best: dict, path: Path = CACHE) -> None:
+    data = load(path)
+    data[key] = {"tile": best["tile"], "rows_per_task": best["rows_per_task"], "ms": best["ms"]}
+    tmp = Path(str(path) + ".tmp")
+    tmp.write_text(json.dumps(data, indent=1, sort_keys=True))
+    os.replace(tmp, path)
+
+
+def main():
+    from . import FP32, FP64, synth
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--config", default="lofar")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp32")
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--out", default=str(CACHE))
+    args = ap.parse_args()
+    cfg = synth.CONFIGS[args.config]
+    ds = synth.make_dataset(cfg, device="cuda:0")
+    prec = FP32 if args.precision == "fp32" else FP64
+    r = tune_grid(ds, prec, steps=args.steps)
+    key = signature(cfg.grid, cfg.planes, cfg.support, prec, cfg.n_rows)
+    store(key, r["best"], Path(args.out))
+    print(json.dumps({"key": key, **r}))
+
+
+if __name__ == "__main__":
+    main()

@@ -73,6 +73,7 @@ struct rick_plan_s {
     bool fft_ok = false;
     bool grid_ok = false;
     Timing timing;
+    SideStream side;
 };
 
 extern "C" {
@@ -117,6 +118,9 @@ int rick_plan_destroy(rick_plan p)
             hipEventDestroy(e.first);
             hipEventDestroy(e.second);
         }
+    if (p->side.s) hipStreamDestroy(p->side.s);
+    if (p->side.fork) hipEventDestroy(p->side.fork);
+    if (p->side.join) hipEventDestroy(p->side.join);
     hipSetDevice(dev);
     delete p;
     return RICK_OK;
@@ -202,7 +206,17 @@ int rick_plan_create(rick_plan *out, const rick_plan_desc *desc)
     char *base = static_cast<char *>(p->dmem);
     p->table_d = reinterpret_cast<double *>(base + o_table);
     if (o_tmp != o_work) p->tmp_plane = base + o_tmp;
-    if (p->grid_ok) grid_work_carve(g, d.max_points, d.precision, base + o_work, &p->work);
+    if (p->grid_ok) {
+        grid_work_carve(g, d.max_points, d.precision, base + o_work, &p->work);
+        // Side stream at the default priority, non-blocking w.r.t. the null stream.
+        if (hipStreamCreateWithFlags(&p->side.s, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&p->side.fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&p->side.join, hipEventDisableTiming) != hipSuccess) {
+            hipFree(p->dmem);
+            hipSetDevice(prev);
+            return set_error(RICK_ERR_DEVICE, "cannot create the plan's side stream");
+        }
+    }
 
     // Host tables -> device (synchronous; plan creation is not on a hot path).
     std::vector<double> h(kMaxTable, 0.0);
@@ -274,7 +288,7 @@ int rick_plan_grid(rick_plan p, unsigned int n, unsigned int F, unsigned int P, 
     RICK_HIP(hipSetDevice(p->d.device));
     RICK_HIP(launch_grid(p->geom, p->work, p->d.precision, p->table_d, n, F, P, uu, vv, ww, vr,
                          vi, wt, grid, accumulate, static_cast<hipStream_t>(stream),
-                         &p->timing));
+                         &p->timing, &p->side));
     return RICK_OK;
 }
 
@@ -291,7 +305,7 @@ int rick_plan_fft(rick_plan p, const void *in, void *out, void *stream)
     // fourier_transform.c:24
     const double norm = 1.0 / (double)((double)p->geom.nx * (double)p->geom.ny);
     RICK_HIP(launch_fft(p->fft, p->d.precision, in, out, p->tmp_plane, norm,
-                        static_cast<hipStream_t>(stream), &p->timing));
+                        static_cast<hipStream_t>(stream), &p->timing, nullptr));
     return RICK_OK;
 }
 
@@ -350,7 +364,7 @@ int rick_plan_grid_stats(rick_plan p, unsigned int *stats, int n)
     RICK_HIP(hipDeviceSynchronize());
     uint32_t meta[4] = {0, 0, 0, 0};
     RICK_HIP(hipMemcpy(meta, p->work.meta, sizeof meta, hipMemcpyDeviceToHost));
-    for (int i = 0; i < n && i < 3; i++) stats[i] = meta[i];
+    for (int i = 0; i < n && i < 4; i++) stats[i] = meta[i];
     return RICK_OK;
 }
 
@@ -360,9 +374,24 @@ int rick_plan_image(rick_plan p, unsigned int n, unsigned int F, unsigned int P,
 {
     int st = rick_plan_grid(p, n, F, P, uu, vv, ww, vr, vi, wt, grid, 0, stream);
     if (st != RICK_OK) return st;
-    st = rick_plan_fft(p, grid, grid, stream);
-    if (st != RICK_OK) return st;
-    return rick_plan_phase(p, grid, re, im, 0, stream);
+    if (!p->fft_ok)
+        return set_error(RICK_ERR_UNSUPPORTED,
+                         "FFT needs power-of-two axes in [16, 8192], got %d x %d", p->geom.nx,
+                         p->geom.ny);
+    if (!re || !im) return set_error(RICK_ERR_INVALID, "null image");
+    if (p->d.phase_on && p->geom.planes > 1 && !(p->d.wmax != p->d.wmin))
+        return set_error(RICK_ERR_INVALID, "wmax == wmin: the w-plane spacing is undefined");
+    // In-place FFT that may leave the rows of each plane permuted; the phase
+    // stage reads them through the same permutation.
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    const double norm = 1.0 / (double)((double)p->geom.nx * (double)p->geom.ny);
+    int row_n1 = 0;
+    RICK_HIP(launch_fft(p->fft, p->d.precision, grid, grid, p->tmp_plane, norm, s, &p->timing,
+                        &row_n1));
+    PhaseGeom ph = p->phase;
+    ph.row_n1 = row_n1;
+    RICK_HIP(launch_phase(ph, p->d.precision, grid, re, im, 0, s, &p->timing));
+    return RICK_OK;
 }
 
 }  // extern "C"

@@ -347,7 +347,7 @@ hipError_t launch_cols(int L, const ColArgs &a, int sets, int planes, const C2<T
 
 template <typename T>
 hipError_t fft_impl(const FftPlan &p, const void *in_v, void *out_v, void *tmp_v,
-                    double scale_d, hipStream_t s, Timing *tm)
+                    double scale_d, hipStream_t s, Timing *tm, int *row_n1)
 {
     using CT = C2<T>;
     const CT *in = static_cast<const CT *>(in_v);
@@ -369,7 +369,23 @@ hipError_t fft_impl(const FftPlan &p, const void *in_v, void *out_v, void *tmp_v
         const int N1 = 64, N2 = ny / 64;
         const ColArgs A{nx, ny, 1, N2, N1, 1, 1, N2};   // rows n2 + n1*N2 -> n2*N1 + k1
         const ColArgs B{nx, ny, 1, N1, 1, N1, 0, N1};   // rows k1 + n2*N1 -> k1 + k2*N1
-        if (in_v != out_v) {
+        if (in_v == out_v && row_n1 != nullptr) {
+            // In place, output rows permuted: A keeps each row set in place
+            // (rows n2 + N2*n1 -> n2 + N2*k1), B transforms blocks of N2
+            // consecutive rows in place, leaving logical row k1 + N1*k2 at
+            // physical row N2*k1 + k2.  No scratch, two full-size launches.
+            const ColArgs At{nx, ny, 1, N2, 1, N2, 1, N2};
+            const ColArgs Bt{nx, ny, N2, 1, N2, 1, 0, N1};
+            tm->begin(RICK_STAGE_FFT_COLS, s);
+            e = launch_cols<T>(N1, At, N2, planes, twy, out, out, s);
+            tm->end(RICK_STAGE_FFT_COLS, s);
+            if (e != hipSuccess) return e;
+            tm->begin(RICK_STAGE_FFT_COLS_B, s);
+            e = launch_cols<T>(N2, Bt, N1, planes, twy, out, out, s);
+            tm->end(RICK_STAGE_FFT_COLS_B, s);
+            if (e != hipSuccess) return e;
+            *row_n1 = N1;
+        } else if (in_v != out_v) {
             tm->begin(RICK_STAGE_FFT_COLS, s);
             e = launch_cols<T>(N1, A, N2, planes, twy, in, out, s);
             tm->end(RICK_STAGE_FFT_COLS, s);
@@ -411,12 +427,13 @@ bool fft_needs_tmp(int precision, int ny)
 }
 
 hipError_t launch_fft(const FftPlan &p, int precision, const void *in, void *out, void *tmp,
-                      double scale, hipStream_t s, Timing *tm)
+                      double scale, hipStream_t s, Timing *tm, int *row_n1)
 {
     Timing off;
     if (tm == nullptr) tm = &off;
-    if (precision == RICK_FP64) return fft_impl<double>(p, in, out, tmp, scale, s, tm);
-    return fft_impl<float>(p, in, out, tmp, scale, s, tm);
+    if (row_n1) *row_n1 = 0;
+    if (precision == RICK_FP64) return fft_impl<double>(p, in, out, tmp, scale, s, tm, row_n1);
+    return fft_impl<float>(p, in, out, tmp, scale, s, tm, row_n1);
 }
 
 }  // namespace rick

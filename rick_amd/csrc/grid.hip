@@ -18,9 +18,10 @@
 //                skipped, w-stacking.cu:517) and writes a record {pos_u,
 //                pos_v, S} to each of its buckets; slots come from one
 //                returning global atomic per distinct key per workgroup.
-//   k_zero_split clears the tiles of split buckets (overwrite mode).
-//   k_tile       one workgroup per task: accumulates its records into an LDS
-//                tile (one lane per footprint cell, LDS atomics), then writes
+//   k_zero_*     clear the tiles of split and of empty buckets (overwrite
+//                mode); empty buckets get no task.
+//   k_tile       one workgroup per task: accumulates its records into an FP64
+//                LDS tile (one lane per footprint cell, LDS atomics), then writes
 //                the tile once with plain, coalesced stores; tasks of split
 //                buckets flush with HBM atomics instead.  Every grid cell is
 //                written by exactly one tile, so no memset of the grid is needed.
@@ -163,17 +164,19 @@ k_count(GridGeom g, uint32_t n, const double *__restrict__ uu, const double *__r
 
 // Scan of the histogram in three kernels: per-block sums, a single-block scan
 // of those, and the per-block downsweep that emits offsets and tasks.
+// Tasks of a bucket with c records (empty buckets have none: their tiles are
+// cleared by k_zero_empty).
 __device__ __forceinline__ uint32_t ntasks_of(uint32_t c, uint32_t rmax)
 {
-    return c == 0 ? 1u : (c + rmax - 1) / rmax;
+    return (c + rmax - 1) / rmax;
 }
 
 __global__ void __launch_bounds__(kThreads)
 k_scan_reduce(GridGeom g, const uint32_t *__restrict__ counts, uint32_t *__restrict__ blocksum,
               uint32_t nscan)
 {
-    __shared__ uint32_t s[3][kThreads];
-    uint32_t r = 0, t = 0, sp = 0;
+    __shared__ uint32_t s[4][kThreads];
+    uint32_t r = 0, t = 0, sp = 0, ne = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const uint32_t b = blockIdx.x * kScanBlock + threadIdx.x * 4 + k;
@@ -182,24 +185,24 @@ k_scan_reduce(GridGeom g, const uint32_t *__restrict__ counts, uint32_t *__restr
             r += c;
             t += nt;
             sp += nt > 1 ? 1u : 0u;
+            ne += c > 0 ? 1u : 0u;
         }
     }
     s[0][threadIdx.x] = r;
     s[1][threadIdx.x] = t;
     s[2][threadIdx.x] = sp;
+    s[3][threadIdx.x] = ne;
     __syncthreads();
     for (int d = kThreads / 2; d > 0; d >>= 1) {
         if ((int)threadIdx.x < d) {
-            s[0][threadIdx.x] += s[0][threadIdx.x + d];
-            s[1][threadIdx.x] += s[1][threadIdx.x + d];
-            s[2][threadIdx.x] += s[2][threadIdx.x + d];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) s[a][threadIdx.x] += s[a][threadIdx.x + d];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        blocksum[blockIdx.x] = s[0][0];
-        blocksum[nscan + blockIdx.x] = s[1][0];
-        blocksum[2 * nscan + blockIdx.x] = s[2][0];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) blocksum[a * nscan + blockIdx.x] = s[a][0];
     }
 }
 
@@ -209,29 +212,29 @@ __global__ void __launch_bounds__(kTopThreads)
 k_scan_top(GridGeom g, uint32_t *__restrict__ blocksum, uint32_t nscan,
            uint32_t *__restrict__ offsets, uint32_t *__restrict__ meta)
 {
-    __shared__ uint32_t s[3][kTopThreads];
-    uint32_t carry[3] = {0, 0, 0};
+    __shared__ uint32_t s[4][kTopThreads];
+    uint32_t carry[4] = {0, 0, 0, 0};
     for (uint32_t c0 = 0; c0 < nscan; c0 += kTopThreads) {
         const uint32_t i = c0 + threadIdx.x;
-        uint32_t v[3];
+        uint32_t v[4];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
+        for (int a = 0; a < 4; ++a) {
             v[a] = i < nscan ? blocksum[a * nscan + i] : 0u;
             s[a][threadIdx.x] = v[a];
         }
         __syncthreads();
         for (int d = 1; d < kTopThreads; d <<= 1) {
-            uint32_t add[3] = {0, 0, 0};
+            uint32_t add[4] = {0, 0, 0, 0};
             if ((int)threadIdx.x >= d)
 #pragma unroll
-                for (int a = 0; a < 3; ++a) add[a] = s[a][threadIdx.x - d];
+                for (int a = 0; a < 4; ++a) add[a] = s[a][threadIdx.x - d];
             __syncthreads();
 #pragma unroll
-            for (int a = 0; a < 3; ++a) s[a][threadIdx.x] += add[a];
+            for (int a = 0; a < 4; ++a) s[a][threadIdx.x] += add[a];
             __syncthreads();
         }
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
+        for (int a = 0; a < 4; ++a) {
             if (i < nscan) blocksum[a * nscan + i] = carry[a] + s[a][threadIdx.x] - v[a];
             carry[a] += s[a][kTopThreads - 1];
         }
@@ -242,13 +245,14 @@ k_scan_top(GridGeom g, uint32_t *__restrict__ blocksum, uint32_t nscan,
         meta[0] = carry[0];
         meta[1] = carry[1];
         meta[2] = carry[2];
+        meta[3] = carry[3];
     }
 }
 
 __global__ void __launch_bounds__(kThreads)
 k_scan_down(GridGeom g, const uint32_t *__restrict__ counts, const uint32_t *__restrict__ blockoff,
             uint32_t nscan, uint32_t *__restrict__ offsets, uint32_t *__restrict__ cursor,
-            uint2 *__restrict__ tasks, uint32_t task_cap, uint32_t *__restrict__ split_list,
+            uint4 *__restrict__ tasks, uint32_t task_cap, uint32_t *__restrict__ split_list,
             uint32_t split_cap)
 {
     __shared__ uint32_t s[3][kThreads];
@@ -287,8 +291,10 @@ k_scan_down(GridGeom g, const uint32_t *__restrict__ counts, const uint32_t *__r
         offsets[b] = ro;
         cursor[b] = ro;
         const uint32_t nt = ntasks_of(c[k], g.rmax);
-        for (uint32_t ch = 0; ch < nt; ++ch)
-            if (to + ch < task_cap) tasks[to + ch] = make_uint2(b, ch);
+        for (uint32_t ch = 0; ch < nt; ++ch) {
+            const uint32_t r0 = ro + ch * g.rmax, r1 = min(r0 + g.rmax, ro + c[k]);
+            if (to + ch < task_cap) tasks[to + ch] = make_uint4(b, r0, r1, nt > 1 ? 1u : 0u);
+        }
         if (nt > 1 && so < split_cap) split_list[so++] = b;
         ro += c[k];
         to += nt;
@@ -310,6 +316,8 @@ k_scatter(GridGeom g, uint32_t n, uint32_t F, uint32_t P, const double *__restri
     }
     __syncthreads();
     const uint32_t base = blockIdx.x * kRowsPerBlock;
+    const bool vec4 = ((F * P) % 4u) == 0u && ((reinterpret_cast<uintptr_t>(vr) |
+                                                reinterpret_cast<uintptr_t>(vi)) & 15u) == 0u;
     double pu[kRowsPerThread], pv[kRowsPerThread], sre[kRowsPerThread], sim[kRowsPerThread];
     uint32_t item[kRowsPerThread][4];
     int nk[kRowsPerThread];
@@ -325,18 +333,39 @@ k_scatter(GridGeom g, uint32_t n, uint32_t F, uint32_t P, const double *__restri
         if (!footprint(g, pu[k], pv[k], ww[i] / g.dw, f)) continue;
         // Channel/polarization sum, w-stacking.cu:512-527 (weights indexed
         // row*P + p, shared across channels; float product first).
-        const size_t vb = (size_t)i * F * P;
+        const uint32_t FP = F * P;
+        const size_t vb = (size_t)i * FP;
         const float *wrow = wt + (size_t)i * P;
         double a_re = 0.0, a_im = 0.0;
-        for (uint32_t fr = 0; fr < F; ++fr) {
-            for (uint32_t p = 0; p < P; ++p) {
-                const size_t q = vb + (size_t)fr * P + p;
-                const float a = vr[q];
-                if (!isnan(a)) {
-                    const float wr = wrow[p] * a;
-                    const float wi = wrow[p] * vi[q];
-                    a_re += (double)wr;
-                    a_im += (double)wi;
+        if (vec4) {
+            // Same sum in the same element order, read 16 bytes at a time.
+            const float4 *r4 = reinterpret_cast<const float4 *>(vr + vb);
+            const float4 *i4 = reinterpret_cast<const float4 *>(vi + vb);
+            for (uint32_t e4 = 0; e4 < FP / 4; ++e4) {
+                const float4 ra = r4[e4], ia = i4[e4];
+                const float rr[4] = {ra.x, ra.y, ra.z, ra.w}, ii[4] = {ia.x, ia.y, ia.z, ia.w};
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    if (!isnan(rr[c])) {
+                        const float wv = wrow[P == 1 ? 0u : (4 * e4 + c) % P];
+                        const float wr = wv * rr[c];
+                        const float wi = wv * ii[c];
+                        a_re += (double)wr;
+                        a_im += (double)wi;
+                    }
+                }
+            }
+        } else {
+            for (uint32_t fr = 0; fr < F; ++fr) {
+                for (uint32_t p = 0; p < P; ++p) {
+                    const size_t q = vb + (size_t)fr * P + p;
+                    const float a = vr[q];
+                    if (!isnan(a)) {
+                        const float wr = wrow[p] * a;
+                        const float wi = wrow[p] * vi[q];
+                        a_re += (double)wr;
+                        a_im += (double)wi;
+                    }
                 }
             }
         }
@@ -405,6 +434,30 @@ k_zero_split(GridGeom g, const uint32_t *__restrict__ split_list, const uint32_t
     }
 }
 
+// Clears the tiles of empty buckets (overwrite mode): every cell of the grid
+// is written either here or by exactly one tile task.
+template <typename T2>
+__global__ void __launch_bounds__(kThreads)
+k_zero_empty(GridGeom g, const uint32_t *__restrict__ counts, T2 *__restrict__ grid)
+{
+    const uint32_t per_plane = (uint32_t)g.tiles_u * (uint32_t)g.tiles_v;
+    for (uint32_t b = blockIdx.x; b < g.nb; b += gridDim.x) {
+        if (counts[b] != 0) continue;
+        const int w = (int)(b / per_plane), rem = (int)(b % per_plane);
+        const int tv = rem / g.tiles_u, tu = rem % g.tiles_u;
+        const int j0 = tu * g.tu, k0 = tv * g.tv;
+        const int nj = min(g.tu, g.nx - j0), nk = min(g.tv, g.ny - k0);
+        T2 z;
+        z.x = 0;
+        z.y = 0;
+        for (int idx = threadIdx.x; idx < nk * g.tu; idx += kThreads) {
+            const int kk = idx / g.tu, jj = idx % g.tu;
+            if (jj >= nj) continue;
+            grid[((size_t)w * g.ny + (size_t)(k0 + kk)) * (size_t)g.nx + (size_t)(j0 + jj)] = z;
+        }
+    }
+}
+
 __device__ __forceinline__ double readlane_f64(double x, int l)
 {
     const uint64_t b = __double_as_longlong(x);
@@ -417,32 +470,28 @@ __device__ __forceinline__ float readlane_f32(float x, int l)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
 }
 
-// Per-task LDS tile accumulation.  PREC selects FP64 LDS atomics or FP32 LDS
-// atomics with an exactly captured rounding error (TwoSum on the value the
-// returning atomic saw) accumulated in a second FP32 array.
+// Per-task LDS tile accumulation.  Both paths accumulate in an FP64 LDS tile
+// with native non-returning ds_add_f64; in the FP32 path the per-cell products
+// are FP32 and the FP64 accumulator holds their sum exactly enough that the
+// tile is rounded to FP32 only once, when it is written to the grid.
 template <int PREC, int TU, int TV>
 __global__ void __launch_bounds__(kTileThreads)
-k_tile(GridGeom g, const uint32_t *__restrict__ offsets, const uint2 *__restrict__ tasks,
-       const uint32_t *__restrict__ meta, const typename RecOf<PREC>::type *__restrict__ rec,
-       const double *__restrict__ table, void *grid_v, int accumulate)
+k_tile(GridGeom g, const uint4 *__restrict__ tasks, const uint32_t *__restrict__ meta,
+       const typename RecOf<PREC>::type *__restrict__ rec, const double *__restrict__ table,
+       void *grid_v, int accumulate)
 {
     using T = typename std::conditional<PREC == RICK_FP64, double, float>::type;
     using T2 = typename std::conditional<PREC == RICK_FP64, double2, float2>::type;
     constexpr int LDU = TU + kPadU;
-    constexpr int NARR = PREC == RICK_FP64 ? 2 : 4;
     constexpr int NWAVES = kTileThreads / 64;
-    __shared__ T lds[NARR * TV * LDU];
+    __shared__ double lds[2 * TV * LDU];
     __shared__ T stab[kMaxTable];
-    T *sre = lds, *sim = lds + TV * LDU;
-    T *cre = lds + 2 * TV * LDU, *cim = lds + 3 * TV * LDU;  // FP32 only
+    double *sre = lds, *sim = lds + TV * LDU;
 
     if (blockIdx.x >= meta[1]) return;
-    const uint2 task = tasks[blockIdx.x];
-    const uint32_t b = task.x;
-    const uint32_t beg = offsets[b], end = offsets[b + 1];
-    const bool split = end - beg > g.rmax;
-    const uint32_t r0 = beg + task.y * g.rmax;
-    const uint32_t r1 = min(r0 + g.rmax, end);
+    const uint4 task = tasks[blockIdx.x];
+    const uint32_t b = task.x, r0 = task.y, r1 = task.z;
+    const bool split = task.w != 0;
 
     const uint32_t per_plane = (uint32_t)g.tiles_u * (uint32_t)g.tiles_v;
     const int w = (int)(b / per_plane), rem = (int)(b % per_plane);
@@ -451,28 +500,21 @@ k_tile(GridGeom g, const uint32_t *__restrict__ offsets, const uint2 *__restrict
     const int nj = min(TU, g.nx - j0), nk = min(TV, g.ny - k0);
     T *grid = static_cast<T *>(grid_v);
 
-    if (r0 >= r1) {
-        // Empty bucket: its cells are zero (overwrite) or unchanged (accumulate).
-        if (accumulate) return;
-        for (int idx = threadIdx.x; idx < nk * TU; idx += kTileThreads) {
-            const int kk = idx / TU, jj = idx % TU;
-            if (jj >= nj) continue;
-            const size_t c = ((size_t)w * g.ny + (size_t)(k0 + kk)) * (size_t)g.nx + (size_t)(j0 + jj);
-            T2 z;
-            z.x = T(0);
-            z.y = T(0);
-            reinterpret_cast<T2 *>(grid)[c] = z;
-        }
-        return;
-    }
-
-    for (int idx = threadIdx.x; idx < NARR * TV * LDU; idx += kTileThreads) lds[idx] = T(0);
+    for (int idx = threadIdx.x; idx < 2 * TV * LDU; idx += kTileThreads) lds[idx] = 0.0;
     for (int idx = threadIdx.x; idx < g.ntable; idx += kTileThreads) stab[idx] = (T)table[idx];
     __syncthreads();
 
     const int lane = (int)(threadIdx.x & 63);
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int S = g.support, KL = g.kl, ncell = S * S;
+    // Footprint cells of this lane (support <= 15: at most 4 per lane).
+    int cdk[4], cdj[4];
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+        const int cell = lane + 64 * it;
+        cdk[it] = cell / S;
+        cdj[it] = cell - cdk[it] * S;
+    }
     // gauss_kernel_norm constants (w-stacking.cu:254-256).
     const double std22 = 1.0 / (2.0 * 1.0 * 1.0);
     const double gnorm = std22 / kRefPi;
@@ -503,9 +545,10 @@ k_tile(GridGeom g, const uint32_t *__restrict__ offsets, const uint2 *__restrict
             }
             Foot f;
             uv_range(g, pu, pv, f);
-            for (int cell = lane; cell < ncell; cell += 64) {
-                const int dk = cell / S, dj = cell - dk * S;
-                const int j = f.gu - KL + dj, k = f.gv - KL + dk;
+#pragma unroll
+            for (int it = 0; it < 4; ++it) {
+                if (lane + 64 * it >= ncell) break;
+                const int j = f.gu - KL + cdj[it], k = f.gv - KL + cdk[it];
                 if (j < f.jmin || j > f.jmax || k < f.kmin || k > f.kmax) continue;
                 if (j < j0 || j >= j0 + nj || k < k0 || k >= k0 + nk) continue;
                 // w-stacking.cu:484, 489, 492-493
@@ -522,21 +565,10 @@ k_tile(GridGeom g, const uint32_t *__restrict__ offsets, const uint2 *__restrict
                 }
                 const int li = (k - k0) * LDU + (j - j0);
                 const T are = s_re * c, aim = s_im * c;
-                if constexpr (PREC == RICK_FP64) {
-                    __hip_atomic_fetch_add(&sre[li], are, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    __hip_atomic_fetch_add(&sim[li], aim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                } else {
-                    // TwoSum of (old, x): the atomic stored fl(old + x); its
-                    // rounding error is recovered exactly and accumulated.
-                    const float o1 = __hip_atomic_fetch_add(&sre[li], are, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    const float o2 = __hip_atomic_fetch_add(&sim[li], aim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    const float s1 = o1 + are, b1 = s1 - o1;
-                    const float e1 = (o1 - (s1 - b1)) + (are - b1);
-                    const float s2 = o2 + aim, b2 = s2 - o2;
-                    const float e2 = (o2 - (s2 - b2)) + (aim - b2);
-                    if (e1 != 0.0f) __hip_atomic_fetch_add(&cre[li], e1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (e2 != 0.0f) __hip_atomic_fetch_add(&cim[li], e2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                }
+                __hip_atomic_fetch_add(&sre[li], (double)are, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(&sim[li], (double)aim, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
     }
@@ -546,11 +578,7 @@ k_tile(GridGeom g, const uint32_t *__restrict__ offsets, const uint2 *__restrict
         const int kk = idx / TU, jj = idx % TU;
         if (jj >= nj) continue;
         const int li = kk * LDU + jj;
-        T vre = sre[li], vim = sim[li];
-        if constexpr (PREC == RICK_FP32) {
-            vre = vre + cre[li];
-            vim = vim + cim[li];
-        }
+        const T vre = (T)sre[li], vim = (T)sim[li];
         const size_t c = ((size_t)w * g.ny + (size_t)(k0 + kk)) * (size_t)g.nx + (size_t)(j0 + jj);
         if (split) {
             if (vre != T(0)) unsafeAtomicAdd(&grid[2 * c], vre);
@@ -580,8 +608,8 @@ hipError_t launch_tile(const GridGeom &g, const GridWork &w, const double *table
 #define RICK_TILE_CASE(A, B)                                                               \
     if (g.tu == A && g.tv == B) {                                                          \
         tm->begin(RICK_STAGE_TILE, s);                                                     \
-        hipLaunchKernelGGL((k_tile<PREC, A, B>), grd, blk, 0, s, g, w.offsets, w.tasks,   \
-                           w.meta, rec, table, grid, accumulate);                          \
+        hipLaunchKernelGGL((k_tile<PREC, A, B>), grd, blk, 0, s, g, w.tasks, w.meta, rec,  \
+                           table, grid, accumulate);                                       \
         tm->end(RICK_STAGE_TILE, s);                                                       \
         return hipGetLastError();                                                          \
     }
@@ -621,8 +649,8 @@ size_t grid_work_bytes(const GridGeom &g, uint32_t max_points, int precision)
     b += align_up(sizeof(uint32_t) * g.nb, 256);           // counts
     b += align_up(sizeof(uint32_t) * g.nb, 256);           // cursor
     b += align_up(sizeof(uint32_t) * (g.nb + 1), 256);     // offsets
-    b += align_up(sizeof(uint32_t) * 3 * nscan, 256);      // blocksum
-    b += align_up(sizeof(uint2) * task_cap, 256);          // tasks
+    b += align_up(sizeof(uint32_t) * 4 * nscan, 256);      // blocksum
+    b += align_up(sizeof(uint4) * task_cap, 256);          // tasks
     b += align_up(sizeof(uint32_t) * split_cap, 256);      // split_list
     b += align_up(sizeof(uint32_t) * 4, 256);              // meta
     b += align_up(rs * rec_cap, 256);                      // records
@@ -644,8 +672,8 @@ void grid_work_carve(const GridGeom &g, uint32_t max_points, int precision, void
     w->counts = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * g.nb));
     w->cursor = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * g.nb));
     w->offsets = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * (g.nb + 1)));
-    w->blocksum = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * 3 * nscan));
-    w->tasks = reinterpret_cast<uint2 *>(take(sizeof(uint2) * task_cap));
+    w->blocksum = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * 4 * nscan));
+    w->tasks = reinterpret_cast<uint4 *>(take(sizeof(uint4) * task_cap));
     w->split_list = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * split_cap));
     w->meta = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * 4));
     w->records = take(rs * rec_cap);
@@ -659,7 +687,7 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
                        const double *table_d, uint32_t n, uint32_t F, uint32_t P,
                        const double *uu, const double *vv, const double *ww, const float *vr,
                        const float *vi, const float *wt, void *grid, int accumulate,
-                       hipStream_t s, Timing *tm)
+                       hipStream_t s, Timing *tm, const SideStream *side)
 {
     Timing off;
     if (tm == nullptr) tm = &off;
@@ -670,6 +698,30 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
     if (n > 0)
         hipLaunchKernelGGL(k_count, dim3(nblk), dim3(kThreads), 0, s, g, n, uu, vv, ww, w.counts);
     tm->end(RICK_STAGE_COUNT_ROWS, s);
+    // Empty tiles depend only on the histogram: clear them on the side stream
+    // while the scan, scatter and tile kernels run (disjoint cells).
+    hipStream_t zs = s;
+    if (!accumulate && side && side->s) {
+        hipError_t e = hipEventRecord(side->fork, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(side->s, side->fork, 0);
+        if (e != hipSuccess) return e;
+        zs = side->s;
+    }
+    if (!accumulate) {
+        const uint32_t zb = min(g.nb, 8192u);
+        tm->begin(RICK_STAGE_ZERO, zs);
+        if (precision == RICK_FP64)
+            hipLaunchKernelGGL(k_zero_empty<double2>, dim3(zb), dim3(kThreads), 0, zs, g, w.counts,
+                               static_cast<double2 *>(grid));
+        else
+            hipLaunchKernelGGL(k_zero_empty<float2>, dim3(zb), dim3(kThreads), 0, zs, g, w.counts,
+                               static_cast<float2 *>(grid));
+        tm->end(RICK_STAGE_ZERO, zs);
+        if (zs != s) {
+            hipError_t e = hipEventRecord(side->join, zs);
+            if (e != hipSuccess) return e;
+        }
+    }
     tm->begin(RICK_STAGE_SCAN, s);
     hipLaunchKernelGGL(k_scan_reduce, dim3(w.nscan), dim3(kThreads), 0, s, g, w.counts,
                        w.blocksum, w.nscan);
@@ -691,19 +743,22 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
                                static_cast<RecF32 *>(w.records));
     }
     tm->end(RICK_STAGE_SCATTER, s);
-    tm->begin(RICK_STAGE_ZERO, s);
     if (!accumulate) {
+        tm->begin(RICK_STAGE_ZERO_SPLIT, s);
         if (precision == RICK_FP64)
             hipLaunchKernelGGL(k_zero_split<double>, dim3(w.split_cap), dim3(kThreads), 0, s, g,
                                w.split_list, w.meta, static_cast<double *>(grid));
         else
             hipLaunchKernelGGL(k_zero_split<float>, dim3(w.split_cap), dim3(kThreads), 0, s, g,
                                w.split_list, w.meta, static_cast<float *>(grid));
+        tm->end(RICK_STAGE_ZERO_SPLIT, s);
     }
-    tm->end(RICK_STAGE_ZERO, s);
-    if (precision == RICK_FP64)
-        return launch_tile<RICK_FP64>(g, w, table_d, grid, accumulate, s, tm);
-    return launch_tile<RICK_FP32>(g, w, table_d, grid, accumulate, s, tm);
+    hipError_t e = precision == RICK_FP64
+                       ? launch_tile<RICK_FP64>(g, w, table_d, grid, accumulate, s, tm)
+                       : launch_tile<RICK_FP32>(g, w, table_d, grid, accumulate, s, tm);
+    if (e != hipSuccess) return e;
+    if (zs != s) return hipStreamWaitEvent(s, side->join, 0);
+    return hipSuccess;
 }
 
 }  // namespace rick

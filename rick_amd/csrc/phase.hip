@@ -33,6 +33,9 @@ k_phase(PhaseGeom g, const void *__restrict__ gridss_v, double *__restrict__ ore
     const int iv = blockIdx.y;
     if (iu >= g.nx) return;
     const size_t pix = (size_t)iv * g.nx + iu;
+    // Physical row of logical row iv in the plane stack.
+    const int prow = g.row_n1 ? (g.ny / g.row_n1) * (iv % g.row_n1) + iv / g.row_n1 : iv;
+    const size_t src = (size_t)prow * g.nx + iu;
     const size_t plane = (size_t)g.nx * g.ny;
     double acc_re = 0.0, acc_im = 0.0;
     const bool phase = g.phase_on && g.planes > 1;
@@ -43,7 +46,7 @@ k_phase(PhaseGeom g, const void *__restrict__ gridss_v, double *__restrict__ ore
         n = sqrt(1 - radius2);
     }
     for (int iw = 0; iw < g.planes; ++iw) {
-        const size_t idx = pix + (size_t)iw * plane;
+        const size_t idx = src + (size_t)iw * plane;
         double p, q;
         if constexpr (PREC == RICK_FP64) {
             const double2 v = reinterpret_cast<const double2 *>(gridss_v)[idx];

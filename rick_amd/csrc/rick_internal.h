@@ -62,10 +62,10 @@ struct GridWork {
     uint32_t *counts;       // [nb]  records per bucket
     uint32_t *cursor;       // [nb]  scatter cursor (starts at the bucket offset)
     uint32_t *offsets;      // [nb+1] exclusive scan of counts
-    uint32_t *blocksum;     // [3*nscan] per scan block: records, tasks, split buckets
-    uint2 *tasks;           // [task_cap] (bucket, chunk) per gridding task
+    uint32_t *blocksum;     // [4*nscan] per scan block: records, tasks, split, non-empty
+    uint4 *tasks;           // [task_cap] (bucket, first record, end record, split)
     uint32_t *split_list;   // [split_cap] buckets handled by more than one task
-    uint32_t *meta;         // [4]: total records, tasks, split buckets
+    uint32_t *meta;         // [4]: total records, tasks, split buckets, non-empty buckets
     void *records;          // [rec_cap] sorted row records
     uint32_t rec_cap, split_cap, task_cap, nscan;
 };
@@ -75,22 +75,30 @@ void grid_work_carve(const GridGeom &g, uint32_t max_points, int precision, void
                      GridWork *w);
 
 // Launches the binned gridder.  Returns a hipError_t.
+// Side stream of a plan: work that only depends on the bucket histogram
+// (clearing empty tiles) is forked onto it and joined back before return.
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
 hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
                        const double *table_d, uint32_t num_points, uint32_t F, uint32_t P,
                        const double *uu, const double *vv, const double *ww,
                        const float *vr, const float *vi, const float *wt, void *grid,
-                       int accumulate, hipStream_t s, Timing *tm);
+                       int accumulate, hipStream_t s, Timing *tm, const SideStream *side);
 
 // FFT: per-plane 2-D backward transform of planes x ny x nx complex, scaled.
 struct FftPlan {
     int planes, nx, ny;
     const void *tw_x, *tw_y;   // device twiddle tables exp(+2 pi i m/N), N entries
 };
-// Long columns (ny above 512) use two passes; an
-// in-place transform then stages each plane through tmp (one plane).
+// Long columns (ny above 512) use two passes.  An in-place transform either
+// stages each plane through tmp (one plane; natural row order), or, when
+// row_n1 is given, runs fully in place and leaves logical row v at physical
+// row (ny/N1)*(v % N1) + v / N1, reporting N1 in *row_n1 (0: natural order).
 bool fft_needs_tmp(int precision, int ny);
 hipError_t launch_fft(const FftPlan &p, int precision, const void *in, void *out, void *tmp,
-                      double scale, hipStream_t s, Timing *tm);
+                      double scale, hipStream_t s, Timing *tm, int *row_n1);
 
 // Phase correction + plane sum.
 struct PhaseGeom {
@@ -98,6 +106,7 @@ struct PhaseGeom {
     double wmin, dw, dwnorm;
     int phase_on;
     const double *ltab, *mtab;  // device: sin(xcoord*resolution) per u / per v index
+    int row_n1;                 // 0, or the FFT's row permutation (see launch_fft)
 };
 hipError_t launch_phase(const PhaseGeom &p, int precision, const void *gridss, double *re,
                         double *im, int accumulate, hipStream_t s, Timing *tm);

@@ -164,9 +164,9 @@ class Plan:
         return {name: (ms[i], cnt[i]) for i, name in enumerate(_lib.STAGES)}
 
     def grid_stats(self) -> dict:
-        s = (ctypes.c_uint * 3)()
-        check(lib().rick_plan_grid_stats(self._h, s, 3))
-        return {"records": s[0], "tasks": s[1], "split_buckets": s[2]}
+        s = (ctypes.c_uint * 4)()
+        check(lib().rick_plan_grid_stats(self._h, s, 4))
+        return {"records": s[0], "tasks": s[1], "split_buckets": s[2], "nonempty_buckets": s[3]}
 
     def new_grid(self):
         torch = _torch()
