@@ -305,7 +305,7 @@ int rick_plan_fft(rick_plan p, const void *in, void *out, void *stream)
     // fourier_transform.c:24
     const double norm = 1.0 / (double)((double)p->geom.nx * (double)p->geom.ny);
     RICK_HIP(launch_fft(p->fft, p->d.precision, in, out, p->tmp_plane, norm,
-                        static_cast<hipStream_t>(stream), &p->timing, nullptr));
+                        static_cast<hipStream_t>(stream), &p->timing));
     return RICK_OK;
 }
 
@@ -372,25 +372,36 @@ int rick_plan_image(rick_plan p, unsigned int n, unsigned int F, unsigned int P,
                     const double *vv, const double *ww, const float *vr, const float *vi,
                     const float *wt, void *grid, double *re, double *im, void *stream)
 {
-    int st = rick_plan_grid(p, n, F, P, uu, vv, ww, vr, vi, wt, grid, 0, stream);
-    if (st != RICK_OK) return st;
+    clear_error();
+    if (!p) return set_error(RICK_ERR_INVALID, "null plan");
+    if (!p->grid_ok) return set_error(RICK_ERR_INVALID, "plan has no gridding capacity");
     if (!p->fft_ok)
         return set_error(RICK_ERR_UNSUPPORTED,
                          "FFT needs power-of-two axes in [16, 8192], got %d x %d", p->geom.nx,
                          p->geom.ny);
-    if (!re || !im) return set_error(RICK_ERR_INVALID, "null image");
+    if (n > p->d.max_points)
+        return set_error(RICK_ERR_INVALID, "num_points %u exceeds plan capacity %u", n,
+                         p->d.max_points);
+    if (n > 0 && (F == 0 || P == 0))
+        return set_error(RICK_ERR_INVALID, "freq_per_chan and polarizations must be >= 1");
+    if (!grid || !re || !im) return set_error(RICK_ERR_INVALID, "null buffer");
+    if (n > 0 && (!uu || !vv || !ww || !vr || !vi || !wt))
+        return set_error(RICK_ERR_INVALID, "null input array");
     if (p->d.phase_on && p->geom.planes > 1 && !(p->d.wmax != p->d.wmin))
         return set_error(RICK_ERR_INVALID, "wmax == wmin: the w-plane spacing is undefined");
-    // In-place FFT that may leave the rows of each plane permuted; the phase
-    // stage reads them through the same permutation.
+    RICK_HIP(hipSetDevice(p->d.device));
     const hipStream_t s = static_cast<hipStream_t>(stream);
+    // Sparse grid: tiles of empty buckets are left unwritten; the row FFT
+    // reads only occupied tiles (bucket counts) and treats the rest as zero.
+    // (A row of more than 256 tiles exceeds the row FFT's occupancy table;
+    // such grids are cleared densely instead.)
+    const bool sparse = p->geom.nx / p->geom.tu <= 256;
+    RICK_HIP(launch_grid(p->geom, p->work, p->d.precision, p->table_d, n, F, P, uu, vv, ww, vr,
+                         vi, wt, grid, 0, s, &p->timing, &p->side, !sparse));
     const double norm = 1.0 / (double)((double)p->geom.nx * (double)p->geom.ny);
-    int row_n1 = 0;
-    RICK_HIP(launch_fft(p->fft, p->d.precision, grid, grid, p->tmp_plane, norm, s, &p->timing,
-                        &row_n1));
-    PhaseGeom ph = p->phase;
-    ph.row_n1 = row_n1;
-    RICK_HIP(launch_phase(ph, p->d.precision, grid, re, im, 0, s, &p->timing));
+    RICK_HIP(launch_image_chain(p->fft, p->d.precision, grid, norm, s, &p->timing,
+                                sparse ? &p->geom : nullptr, sparse ? p->work.counts : nullptr,
+                                p->phase, re, im));
     return RICK_OK;
 }
 

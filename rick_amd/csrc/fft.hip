@@ -24,6 +24,7 @@
 // sub-transform uses them with stride N/L.  Per butterfly only W^k, W^2k,
 // W^4k, W^8k are loaded; the other powers are products of those.
 #include "rick_internal.h"
+#include "rick_math.h"
 
 #include <algorithm>
 
@@ -181,6 +182,11 @@ struct Lines {
     size_t in_el, out_el;       // offset between consecutive elements of a line
     int nvalid;                 // lines of this workgroup that exist
     int tw_set;                 // twiddle W_N^(n*tw_set) after the last pass (0: none)
+    // Optional tile occupancy of the input (rows only): element n of line g
+    // is read only if occ[g * occ_stride + n / occ_tu] is non-zero, else it
+    // is taken as zero (cells of empty gridder tiles are never written).
+    const unsigned char *occ;
+    int occ_stride, occ_tu;
 };
 
 // Butterfly b of a pass with L/R butterflies per line: rows map consecutive
@@ -192,10 +198,30 @@ __device__ __forceinline__ void bmap(int b, int &g, int &j)
     else { g = b % G; j = b / G; }
 }
 
-template <typename T, int L, int G, bool ROWS, int P>
+// Where the last pass delivers element n of line g (register slot idx):
+// by default to global memory, after the optional four-step twiddle and the
+// output scale.
+template <typename T>
+struct GlobalSink {
+    C2<T> *gout;
+    const Lines *ln;
+    const C2<T> *tw;
+    T scale;
+    __device__ __forceinline__ void operator()(int g, int n, int idx, C2<T> o) const
+    {
+        (void)idx;
+        if (ln->tw_set) o = cmul(o, tw[(size_t)n * ln->tw_set]);
+        o.x *= scale;
+        o.y *= scale;
+        if (g < ln->nvalid)
+            gout[ln->out_base + (size_t)g * ln->out_line + (size_t)n * ln->out_el] = o;
+    }
+};
+
+template <typename T, int L, int G, bool ROWS, int P, typename Sink>
 __device__ __forceinline__ void pass(C2<T> *vals, int tid, const Lines &ln, int ts,
                                      const C2<T> *__restrict__ tw, C2<T> *lds, const C2<T> *gin,
-                                     C2<T> *gout, T scale)
+                                     Sink &sink)
 {
     constexpr int NP = npasses(L);
     if constexpr (P < NP) {
@@ -214,9 +240,10 @@ __device__ __forceinline__ void pass(C2<T> *vals, int tid, const Lines &ln, int 
                 if constexpr (from_lds) {
                     v[r] = ROWS ? lds[g * LP + lpad(n)] : lds[n * G + g];
                 } else {
-                    v[r] = g < ln.nvalid ? gin[ln.in_base + (size_t)g * ln.in_line +
-                                               (size_t)n * ln.in_el]
-                                         : C2<T>{T(0), T(0)};
+                    const bool live = g < ln.nvalid &&
+                                      (ln.occ == nullptr || ln.occ[g * ln.occ_stride + n / ln.occ_tu]);
+                    v[r] = live ? gin[ln.in_base + (size_t)g * ln.in_line + (size_t)n * ln.in_el]
+                                : C2<T>{T(0), T(0)};
                 }
             }
         }
@@ -236,17 +263,12 @@ __device__ __forceinline__ void pass(C2<T> *vals, int tid, const Lines &ln, int 
                     if (ROWS) lds[g * LP + lpad(n)] = v[r];
                     else lds[n * G + g] = v[r];
                 } else {
-                    C2<T> o = v[r];
-                    if (ln.tw_set) o = cmul(o, tw[(size_t)n * ln.tw_set]);
-                    o.x *= scale;
-                    o.y *= scale;
-                    if (g < ln.nvalid)
-                        gout[ln.out_base + (size_t)g * ln.out_line + (size_t)n * ln.out_el] = o;
+                    sink(g, n, i * R + r, v[r]);
                 }
             }
         }
         if constexpr (to_lds) __syncthreads();
-        pass<T, L, G, ROWS, P + 1>(vals, tid, ln, ts, tw, lds, gin, gout, scale);
+        pass<T, L, G, ROWS, P + 1>(vals, tid, ln, ts, tw, lds, gin, sink);
     }
 }
 
@@ -257,14 +279,27 @@ constexpr int lds_elems(int L, bool rows)
 {
     return rows ? lines_per_wg(L) * (lpad(L) + 1) : lines_per_wg(L) * L;
 }
+// Occupancy flags per workgroup: G rows x (L/32) tiles of >= 32 cells.
+constexpr int oc_bytes(int L) { return lines_per_wg(L) * (L / 32 > 0 ? L / 32 : 1); }
+
+// Tile occupancy of a sparse grid (gridder buckets): counts of the bucket
+// (plane, tile row, tile column); zero means the tile was never written.
+struct Occupancy {
+    const uint32_t *counts;
+    int tu, tv, tiles_u, tiles_v, ny;
+};
 
 // Rows: workgroup w handles rows w*G .. w*G+G-1 of the planes*ny rows.
 template <typename T, int L>
 __global__ void __launch_bounds__(L >= 8192 ? 512 : 256)
-k_fft_rows(int rows_total, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gout, T scale)
+k_fft_rows(int rows_total, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gout, T scale,
+           Occupancy oc)
 {
     constexpr int G = lines_per_wg(L);
+    constexpr int NT = G * L / kE;
+    constexpr int MAXT = 256;  // tiles per row held in LDS (tile width >= 32, L <= 8192)
     __shared__ C2<T> lds[lds_elems(L, true)];
+    __shared__ unsigned char occ[oc_bytes(L)];
     Lines ln;
     const size_t row0 = (size_t)blockIdx.x * G;
     ln.in_base = ln.out_base = row0 * L;
@@ -272,8 +307,26 @@ k_fft_rows(int rows_total, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T>
     ln.in_el = ln.out_el = 1;
     ln.nvalid = (int)min((size_t)G, (size_t)rows_total - row0);
     ln.tw_set = 0;
+    ln.occ = nullptr;
+    if (oc.counts) {
+        for (int i = threadIdx.x; i < G * oc.tiles_u && oc.tiles_u <= MAXT; i += NT) {
+            const int g = i / oc.tiles_u, t = i % oc.tiles_u;
+            const size_t row = row0 + g;
+            unsigned char live = 0;
+            if (g < ln.nvalid) {
+                const int plane = (int)(row / oc.ny), v = (int)(row % oc.ny);
+                live = oc.counts[((size_t)plane * oc.tiles_v + v / oc.tv) * oc.tiles_u + t] != 0;
+            }
+            occ[i] = live;
+        }
+        __syncthreads();
+        ln.occ = occ;
+        ln.occ_stride = oc.tiles_u;
+        ln.occ_tu = oc.tu;
+    }
     C2<T> vals[kE];
-    pass<T, L, G, true, 0>(vals, threadIdx.x, ln, 1, tw, lds, gin, gout, scale);
+    GlobalSink<T> sink{gout, &ln, tw, scale};
+    pass<T, L, G, true, 0>(vals, threadIdx.x, ln, 1, tw, lds, gin, sink);
 }
 
 // Columns: workgroup (x: strip of G columns, y: row set s, z: plane).
@@ -302,20 +355,108 @@ k_fft_cols(ColArgs a, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gou
     ln.out_el = (size_t)a.out_stride * a.nx;
     ln.nvalid = min(G, a.nx - c0);
     ln.tw_set = a.tw_by_set ? s : 0;
+    ln.occ = nullptr;
     C2<T> vals[kE];
-    pass<T, L, G, false, 0>(vals, threadIdx.x, ln, a.ts, tw, lds, gin, gout, scale);
+    GlobalSink<T> sink{gout, &ln, tw, scale};
+    pass<T, L, G, false, 0>(vals, threadIdx.x, ln, a.ts, tw, lds, gin, sink);
+}
+
+// Last column pass fused with the phase correction and plane sum
+// (phase_correction.cu:185-250): the workgroup transforms its column strip of
+// every w-plane in turn and accumulates the phase-rotated results of each
+// output pixel in registers; only the image is written.  Output element n of
+// set s is image row  s*vs_mul + n*v_stride.
+template <typename T>
+struct PhaseSink {
+    double are[kE], aim[kE];
+    const double *ltab, *mtab;
+    double wterm, dwnorm;
+    int u0, s, vs_mul, v_stride, nvalid;
+    bool phase;
+    __device__ __forceinline__ void operator()(int g, int n, int idx, C2<T> o)
+    {
+        if (g >= nvalid) return;
+        if (!phase) {
+            are[idx] += (double)o.x;
+            aim[idx] += (double)o.y;
+            return;
+        }
+        const int u = u0 + g, v = s * vs_mul + n * v_stride;
+        const double nf = pixel_n(ltab[u], mtab[v]);
+        if constexpr (sizeof(T) == 8) phase_add_f64(o.x, o.y, wterm, nf, dwnorm, are[idx], aim[idx]);
+        else phase_add_f32(o.x, o.y, wterm, nf, dwnorm, are[idx], aim[idx]);
+    }
+};
+
+template <typename T, int L>
+__global__ void __launch_bounds__(256, 2)
+k_fft_cols_phase(ColArgs a, int planes, const C2<T> *__restrict__ tw,
+                 const C2<T> *__restrict__ grid, PhaseGeom pg, int vs_mul, int v_stride,
+                 double *__restrict__ ore, double *__restrict__ oim)
+{
+    constexpr int G = lines_per_wg(L);
+    constexpr int NT = G * L / kE;
+    __shared__ C2<T> lds[lds_elems(L, false)];
+    const int c0 = blockIdx.x * G;
+    const int s = blockIdx.y;
+    PhaseSink<T> sink;
+#pragma unroll
+    for (int i = 0; i < kE; ++i) sink.are[i] = sink.aim[i] = 0.0;
+    sink.ltab = pg.ltab;
+    sink.mtab = pg.mtab;
+    sink.dwnorm = pg.dwnorm;
+    sink.u0 = c0;
+    sink.s = s;
+    sink.vs_mul = vs_mul;
+    sink.v_stride = v_stride;
+    sink.nvalid = min(G, a.nx - c0);
+    sink.phase = pg.phase_on && planes > 1;
+    Lines ln;
+    ln.in_line = ln.out_line = 1;
+    ln.in_el = (size_t)a.in_stride * a.nx;
+    ln.out_el = 0;
+    ln.nvalid = sink.nvalid;
+    ln.tw_set = 0;
+    ln.occ = nullptr;
+    for (int iw = 0; iw < planes; ++iw) {
+        const size_t plane = (size_t)iw * (size_t)a.nx * (size_t)a.ny;
+        ln.in_base = plane + (size_t)s * a.in_set_mul * (size_t)a.nx + c0;
+        ln.out_base = 0;
+        sink.wterm = pg.wmin + iw * pg.dw;  // phase_correction.cu:200
+        C2<T> vals[kE];
+        pass<T, L, G, false, 0>(vals, threadIdx.x, ln, a.ts, tw, lds, grid, sink);
+        __syncthreads();  // LDS is reused by the next plane
+    }
+    // Write the accumulated pixels: the last pass's element layout.
+    constexpr int P = npasses(L) - 1, R = radix_of(L, P), Ns = ns_of(L, P), LR = L / R,
+                  BPT = kE / R;
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+        int g, j;
+        bmap<false, LR, G>(threadIdx.x + i * NT, g, j);
+        const int dbase = (j / Ns) * Ns * R + j % Ns;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int n = dbase + r * Ns;
+            if (g < sink.nvalid) {
+                const size_t pix = (size_t)(s * vs_mul + n * v_stride) * a.nx + (c0 + g);
+                ore[pix] = sink.are[i * R + r];
+                oim[pix] = sink.aim[i * R + r];
+            }
+        }
+    }
 }
 
 // Launch helpers dispatching on the (power-of-two) line length.
 template <typename T>
 hipError_t launch_rows(int L, int rows_total, const C2<T> *tw, const C2<T> *in, C2<T> *out,
-                       T scale, hipStream_t s)
+                       T scale, hipStream_t s, const Occupancy &oc)
 {
 #define RICK_ROWS(LL)                                                                      \
     case LL: {                                                                             \
         constexpr int G = lines_per_wg(LL);                                                \
         hipLaunchKernelGGL((k_fft_rows<T, LL>), dim3((rows_total + G - 1) / G),            \
-                           dim3(G * LL / kE), 0, s, rows_total, tw, in, out, scale);       \
+                           dim3(G * LL / kE), 0, s, rows_total, tw, in, out, scale, oc);   \
         return hipGetLastError();                                                          \
     }
     switch (L) {
@@ -346,8 +487,28 @@ hipError_t launch_cols(int L, const ColArgs &a, int sets, int planes, const C2<T
 }
 
 template <typename T>
+hipError_t launch_cols_phase(int L, const ColArgs &a, int sets, int planes, const C2<T> *tw,
+                             const C2<T> *grid, const PhaseGeom &pg, int vs_mul, int v_stride,
+                             double *re, double *im, hipStream_t s)
+{
+#define RICK_CPH(LL)                                                                         \
+    case LL: {                                                                               \
+        constexpr int G = lines_per_wg(LL);                                                  \
+        hipLaunchKernelGGL((k_fft_cols_phase<T, LL>), dim3((a.nx + G - 1) / G, sets, 1),     \
+                           dim3(G * LL / kE), 0, s, a, planes, tw, grid, pg, vs_mul,         \
+                           v_stride, re, im);                                                \
+        return hipGetLastError();                                                            \
+    }
+    switch (L) {
+        RICK_CPH(16) RICK_CPH(32) RICK_CPH(64) RICK_CPH(128) RICK_CPH(256) RICK_CPH(512)
+        default: return hipErrorInvalidValue;
+    }
+#undef RICK_CPH
+}
+
+template <typename T>
 hipError_t fft_impl(const FftPlan &p, const void *in_v, void *out_v, void *tmp_v,
-                    double scale_d, hipStream_t s, Timing *tm, int *row_n1)
+                    double scale_d, hipStream_t s, Timing *tm)
 {
     using CT = C2<T>;
     const CT *in = static_cast<const CT *>(in_v);
@@ -369,23 +530,7 @@ hipError_t fft_impl(const FftPlan &p, const void *in_v, void *out_v, void *tmp_v
         const int N1 = 64, N2 = ny / 64;
         const ColArgs A{nx, ny, 1, N2, N1, 1, 1, N2};   // rows n2 + n1*N2 -> n2*N1 + k1
         const ColArgs B{nx, ny, 1, N1, 1, N1, 0, N1};   // rows k1 + n2*N1 -> k1 + k2*N1
-        if (in_v == out_v && row_n1 != nullptr) {
-            // In place, output rows permuted: A keeps each row set in place
-            // (rows n2 + N2*n1 -> n2 + N2*k1), B transforms blocks of N2
-            // consecutive rows in place, leaving logical row k1 + N1*k2 at
-            // physical row N2*k1 + k2.  No scratch, two full-size launches.
-            const ColArgs At{nx, ny, 1, N2, 1, N2, 1, N2};
-            const ColArgs Bt{nx, ny, N2, 1, N2, 1, 0, N1};
-            tm->begin(RICK_STAGE_FFT_COLS, s);
-            e = launch_cols<T>(N1, At, N2, planes, twy, out, out, s);
-            tm->end(RICK_STAGE_FFT_COLS, s);
-            if (e != hipSuccess) return e;
-            tm->begin(RICK_STAGE_FFT_COLS_B, s);
-            e = launch_cols<T>(N2, Bt, N1, planes, twy, out, out, s);
-            tm->end(RICK_STAGE_FFT_COLS_B, s);
-            if (e != hipSuccess) return e;
-            *row_n1 = N1;
-        } else if (in_v != out_v) {
+        if (in_v != out_v) {
             tm->begin(RICK_STAGE_FFT_COLS, s);
             e = launch_cols<T>(N1, A, N2, planes, twy, in, out, s);
             tm->end(RICK_STAGE_FFT_COLS, s);
@@ -413,8 +558,48 @@ hipError_t fft_impl(const FftPlan &p, const void *in_v, void *out_v, void *tmp_v
     }
     // ---- rows (u axis, length nx), in place on out, scaled ----
     tm->begin(RICK_STAGE_FFT_ROWS, s);
-    e = launch_rows<T>(nx, planes * ny, twx, out, out, (T)scale_d, s);
+    e = launch_rows<T>(nx, planes * ny, twx, out, out, (T)scale_d, s, Occupancy{});
     tm->end(RICK_STAGE_FFT_ROWS, s);
+    return e;
+}
+
+// Image chain: in place, rows first reading only occupied tiles of a sparse
+// grid, then (for ny > 512) four-step pass A in place with rows left in
+// transposed order, then the last column pass fused with the phase
+// correction and plane sum, writing the image.
+template <typename T>
+hipError_t image_chain_impl(const FftPlan &p, void *grid_v, double scale_d, hipStream_t s,
+                            Timing *tm, const Occupancy &oc, const PhaseGeom &pg, double *re,
+                            double *im)
+{
+    using CT = C2<T>;
+    CT *grid = static_cast<CT *>(grid_v);
+    const CT *twx = static_cast<const CT *>(p.tw_x);
+    const CT *twy = static_cast<const CT *>(p.tw_y);
+    const int nx = p.nx, ny = p.ny, planes = p.planes;
+    tm->begin(RICK_STAGE_FFT_ROWS, s);
+    hipError_t e = launch_rows<T>(nx, planes * ny, twx, grid, grid, (T)scale_d, s, oc);
+    tm->end(RICK_STAGE_FFT_ROWS, s);
+    if (e != hipSuccess) return e;
+    if (!fft_needs_tmp(sizeof(T) == 8 ? RICK_FP64 : RICK_FP32, ny)) {
+        const ColArgs a{nx, ny, 0, 1, 0, 1, 0, 1};
+        tm->begin(RICK_STAGE_PHASE, s);
+        e = launch_cols_phase<T>(ny, a, 1, planes, twy, grid, pg, 0, 1, re, im, s);
+        tm->end(RICK_STAGE_PHASE, s);
+        return e;
+    }
+    const int N1 = 64, N2 = ny / 64;
+    // A: rows n2 + N2*n1 -> n2 + N2*k1 (in place), twiddle W_N^(k1*n2).
+    const ColArgs At{nx, ny, 1, N2, 1, N2, 1, N2};
+    // B: rows N2*k1 + n2 -> X[k1 + N1*k2], image row k1 + N1*k2.
+    const ColArgs Bt{nx, ny, N2, 1, N2, 1, 0, N1};
+    tm->begin(RICK_STAGE_FFT_COLS, s);
+    e = launch_cols<T>(N1, At, N2, planes, twy, grid, grid, s);
+    tm->end(RICK_STAGE_FFT_COLS, s);
+    if (e != hipSuccess) return e;
+    tm->begin(RICK_STAGE_PHASE, s);
+    e = launch_cols_phase<T>(N2, Bt, N1, planes, twy, grid, pg, 1, N1, re, im, s);
+    tm->end(RICK_STAGE_PHASE, s);
     return e;
 }
 
@@ -427,13 +612,34 @@ bool fft_needs_tmp(int precision, int ny)
 }
 
 hipError_t launch_fft(const FftPlan &p, int precision, const void *in, void *out, void *tmp,
-                      double scale, hipStream_t s, Timing *tm, int *row_n1)
+                      double scale, hipStream_t s, Timing *tm)
 {
     Timing off;
     if (tm == nullptr) tm = &off;
-    if (row_n1) *row_n1 = 0;
-    if (precision == RICK_FP64) return fft_impl<double>(p, in, out, tmp, scale, s, tm, row_n1);
-    return fft_impl<float>(p, in, out, tmp, scale, s, tm, row_n1);
+    if (precision == RICK_FP64) return fft_impl<double>(p, in, out, tmp, scale, s, tm);
+    return fft_impl<float>(p, in, out, tmp, scale, s, tm);
+}
+
+hipError_t launch_image_chain(const FftPlan &p, int precision, void *grid, double scale,
+                              hipStream_t s, Timing *tm, const GridGeom *occ_geom,
+                              const uint32_t *occ_counts, const PhaseGeom &pg, double *re,
+                              double *im)
+{
+    Timing off;
+    if (tm == nullptr) tm = &off;
+    Occupancy oc{};
+    if (occ_geom) {
+        if (!occ_counts || p.nx / occ_geom->tu > 256) return hipErrorInvalidValue;
+        oc.counts = occ_counts;
+        oc.tu = occ_geom->tu;
+        oc.tv = occ_geom->tv;
+        oc.tiles_u = occ_geom->tiles_u;
+        oc.tiles_v = occ_geom->tiles_v;
+        oc.ny = occ_geom->ny;
+    }
+    if (precision == RICK_FP64)
+        return image_chain_impl<double>(p, grid, scale, s, tm, oc, pg, re, im);
+    return image_chain_impl<float>(p, grid, scale, s, tm, oc, pg, re, im);
 }
 
 }  // namespace rick

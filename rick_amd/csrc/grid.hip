@@ -21,7 +21,7 @@
 //   k_zero_*     clear the tiles of split and of empty buckets (overwrite
 //                mode); empty buckets get no task.
 //   k_tile       one workgroup per task: accumulates its records into an FP64
-//                LDS tile (one lane per footprint cell, LDS atomics), then writes
+//                LDS tile (one lane per record, LDS atomics), then writes
 //                the tile once with plain, coalesced stores; tasks of split
 //                buckets flush with HBM atomics instead.  Every grid cell is
 //                written by exactly one tile, so no memset of the grid is needed.
@@ -458,23 +458,15 @@ k_zero_empty(GridGeom g, const uint32_t *__restrict__ counts, T2 *__restrict__ g
     }
 }
 
-__device__ __forceinline__ double readlane_f64(double x, int l)
-{
-    const uint64_t b = __double_as_longlong(x);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
-__device__ __forceinline__ float readlane_f32(float x, int l)
-{
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
-}
-
-// Per-task LDS tile accumulation.  Both paths accumulate in an FP64 LDS tile
-// with native non-returning ds_add_f64; in the FP32 path the per-cell products
-// are FP32 and the FP64 accumulator holds their sum exactly enough that the
-// tile is rounded to FP32 only once, when it is written to the grid.
-template <int PREC, int TU, int TV>
+// Per-task LDS tile accumulation.  Each lane owns one record at a time: it
+// forms the S u-weights K[jKer] and, per footprint row, the v-weight K[kKer]
+// (the reference's index arithmetic, evaluated 2S times per record instead
+// of twice per cell), then adds S*(K[jKer]*K[kKer]) to the S x S cells with
+// non-returning LDS atomics, so the 64 lanes of a wave stream independent
+// updates.  Both paths accumulate in an FP64 LDS tile; in the FP32 path the
+// per-cell products are FP32 and the tile is rounded to FP32 once, when it is
+// written to the grid.  SMAX >= support bounds the unrolled footprint loops.
+template <int PREC, int TU, int TV, int SMAX>
 __global__ void __launch_bounds__(kTileThreads)
 k_tile(GridGeom g, const uint4 *__restrict__ tasks, const uint32_t *__restrict__ meta,
        const typename RecOf<PREC>::type *__restrict__ rec, const double *__restrict__ table,
@@ -483,7 +475,6 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, const uint32_t *__restrict__
     using T = typename std::conditional<PREC == RICK_FP64, double, float>::type;
     using T2 = typename std::conditional<PREC == RICK_FP64, double2, float2>::type;
     constexpr int LDU = TU + kPadU;
-    constexpr int NWAVES = kTileThreads / 64;
     __shared__ double lds[2 * TV * LDU];
     __shared__ T stab[kMaxTable];
     double *sre = lds, *sim = lds + TV * LDU;
@@ -504,70 +495,72 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, const uint32_t *__restrict__
     for (int idx = threadIdx.x; idx < g.ntable; idx += kTileThreads) stab[idx] = (T)table[idx];
     __syncthreads();
 
-    const int lane = (int)(threadIdx.x & 63);
-    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int S = g.support, KL = g.kl, ncell = S * S;
-    // Footprint cells of this lane (support <= 15: at most 4 per lane).
-    int cdk[4], cdj[4];
-#pragma unroll
-    for (int it = 0; it < 4; ++it) {
-        const int cell = lane + 64 * it;
-        cdk[it] = cell / S;
-        cdj[it] = cell - cdk[it] * S;
-    }
+    const int S = g.support, KL = g.kl;
+    const bool hiprec = g.kind == RICK_GAUSS_HI_PRECISION;
     // gauss_kernel_norm constants (w-stacking.cu:254-256).
     const double std22 = 1.0 / (2.0 * 1.0 * 1.0);
     const double gnorm = std22 / kRefPi;
+    // Cells of this tile a footprint may touch, intersected with the grid.
+    const int jlo = j0, jhi = j0 + nj - 1, klo = k0, khi = k0 + nk - 1;
 
-    for (uint32_t base = r0 + (uint32_t)wave * 64u; base < r1; base += NWAVES * 64u) {
-        // Each lane fetches one record; rows are then broadcast one by one.
-        const uint32_t my = base + (uint32_t)lane;
-        double mpu = 0.0, mpv = 0.0;
-        T msr = T(0), msi = T(0);
-        if (my < r1) {
-            const auto r = rec[my];
-            mpu = r.pu;
-            mpv = r.pv;
-            msr = r.sre;
-            msi = r.sim;
-        }
-        const int nrows = (int)min(64u, r1 - base);
-        for (int q = 0; q < nrows; ++q) {
-            const double pu = readlane_f64(mpu, q);
-            const double pv = readlane_f64(mpv, q);
-            T s_re, s_im;
-            if constexpr (PREC == RICK_FP64) {
-                s_re = readlane_f64(msr, q);
-                s_im = readlane_f64(msi, q);
-            } else {
-                s_re = readlane_f32(msr, q);
-                s_im = readlane_f32(msi, q);
-            }
-            Foot f;
-            uv_range(g, pu, pv, f);
+    for (uint32_t r = r0 + threadIdx.x; r < r1; r += kTileThreads) {
+        const auto rc = rec[r];
+        const double pu = rc.pu, pv = rc.pv;
+        const T s_re = rc.sre, s_im = rc.sim;
+        Foot f;
+        uv_range(g, pu, pv, f);
+        const int ja = max(f.jmin, jlo), jb = min(f.jmax, jhi);
+        const int ka = max(f.kmin, klo), kb = min(f.kmax, khi);
+        const int ju0 = f.gu - KL, kv0 = f.gv - KL;
+        // u-weights of the footprint columns (w-stacking.cu:489, 492).
+        T wu[SMAX];
+        double ud[SMAX];
+        uint32_t umask = 0;
 #pragma unroll
-            for (int it = 0; it < 4; ++it) {
-                if (lane + 64 * it >= ncell) break;
-                const int j = f.gu - KL + cdj[it], k = f.gv - KL + cdk[it];
-                if (j < f.jmin || j > f.jmax || k < f.kmin || k > f.kmax) continue;
-                if (j < j0 || j >= j0 + nj || k < k0 || k >= k0 + nk) continue;
-                // w-stacking.cu:484, 489, 492-493
-                const double v_dist = (double)k + 0.5 - pv;
+        for (int dj = 0; dj < SMAX; ++dj) {
+            const int j = ju0 + dj;
+            wu[dj] = T(0);
+            ud[dj] = 0.0;
+            if (dj < S && j >= ja && j <= jb) {
                 const double u_dist = (double)j + 0.5 - pu;
-                T c;
-                if (g.kind == RICK_GAUSS_HI_PRECISION) {
-                    c = (T)(gnorm * exp(-((u_dist * u_dist) + (v_dist * v_dist)) * std22));
-                } else {
-                    const int jKer = (int)(kOversample * (fabs(u_dist + (double)KL)));
-                    const int kKer = (int)(kOversample * (fabs(v_dist + (double)KL)));
-                    if (jKer < 0 || jKer >= g.ntable || kKer < 0 || kKer >= g.ntable) continue;
-                    c = stab[jKer] * stab[kKer];
+                ud[dj] = u_dist;
+                const int jKer = (int)(kOversample * (fabs(u_dist + (double)KL)));
+                if (hiprec) {
+                    umask |= 1u << dj;
+                } else if (jKer >= 0 && jKer < g.ntable) {
+                    wu[dj] = stab[jKer];
+                    umask |= 1u << dj;
                 }
-                const int li = (k - k0) * LDU + (j - j0);
+            }
+        }
+        if (umask == 0) continue;
+#pragma unroll
+        for (int dk = 0; dk < SMAX; ++dk) {
+            const int k = kv0 + dk;
+            if (dk >= S || k < ka || k > kb) continue;
+            // w-stacking.cu:484, 493
+            const double v_dist = (double)k + 0.5 - pv;
+            T wv = T(0);
+            if (!hiprec) {
+                const int kKer = (int)(kOversample * (fabs(v_dist + (double)KL)));
+                if (kKer < 0 || kKer >= g.ntable) continue;
+                wv = stab[kKer];
+            }
+            // LDS index of footprint column 0 on this row; only columns with
+            // their umask bit set (inside the tile) are ever accessed.
+            const int rowbase = (k - k0) * LDU + (ju0 - j0);
+#pragma unroll
+            for (int dj = 0; dj < SMAX; ++dj) {
+                if (!((umask >> dj) & 1u)) continue;
+                // conv_weight = K[jKer] * K[kKer] (w-stacking.cu:499/505), or
+                // gauss_kernel_norm per cell (w-stacking.cu:496).
+                const T c = hiprec ? (T)(gnorm * exp(-((ud[dj] * ud[dj]) + (v_dist * v_dist)) *
+                                                     std22))
+                                   : wu[dj] * wv;
                 const T are = s_re * c, aim = s_im * c;
-                __hip_atomic_fetch_add(&sre[li], (double)are, __ATOMIC_RELAXED,
+                __hip_atomic_fetch_add(&sre[rowbase + dj], (double)are, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-                __hip_atomic_fetch_add(&sim[li], (double)aim, __ATOMIC_RELAXED,
+                __hip_atomic_fetch_add(&sim[rowbase + dj], (double)aim, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
@@ -598,9 +591,9 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, const uint32_t *__restrict__
     }
 }
 
-template <int PREC>
-hipError_t launch_tile(const GridGeom &g, const GridWork &w, const double *table, void *grid,
-                       int accumulate, hipStream_t s, Timing *tm)
+template <int PREC, int SMAX>
+hipError_t launch_tile_s(const GridGeom &g, const GridWork &w, const double *table, void *grid,
+                         int accumulate, hipStream_t s, Timing *tm)
 {
     using Rec = typename RecOf<PREC>::type;
     const Rec *rec = static_cast<const Rec *>(w.records);
@@ -608,8 +601,8 @@ hipError_t launch_tile(const GridGeom &g, const GridWork &w, const double *table
 #define RICK_TILE_CASE(A, B)                                                               \
     if (g.tu == A && g.tv == B) {                                                          \
         tm->begin(RICK_STAGE_TILE, s);                                                     \
-        hipLaunchKernelGGL((k_tile<PREC, A, B>), grd, blk, 0, s, g, w.tasks, w.meta, rec,  \
-                           table, grid, accumulate);                                       \
+        hipLaunchKernelGGL((k_tile<PREC, A, B, SMAX>), grd, blk, 0, s, g, w.tasks, w.meta, \
+                           rec, table, grid, accumulate);                                  \
         tm->end(RICK_STAGE_TILE, s);                                                       \
         return hipGetLastError();                                                          \
     }
@@ -620,6 +613,16 @@ hipError_t launch_tile(const GridGeom &g, const GridWork &w, const double *table
     RICK_TILE_CASE(32, 64)
 #undef RICK_TILE_CASE
     return hipErrorInvalidValue;
+}
+
+// Footprint loops are unrolled to 7 (supports 1-7), 9, or 15 (11-15).
+template <int PREC>
+hipError_t launch_tile(const GridGeom &g, const GridWork &w, const double *table, void *grid,
+                       int accumulate, hipStream_t s, Timing *tm)
+{
+    if (g.support <= 7) return launch_tile_s<PREC, 7>(g, w, table, grid, accumulate, s, tm);
+    if (g.support <= 9) return launch_tile_s<PREC, 9>(g, w, table, grid, accumulate, s, tm);
+    return launch_tile_s<PREC, 15>(g, w, table, grid, accumulate, s, tm);
 }
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -687,7 +690,7 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
                        const double *table_d, uint32_t n, uint32_t F, uint32_t P,
                        const double *uu, const double *vv, const double *ww, const float *vr,
                        const float *vi, const float *wt, void *grid, int accumulate,
-                       hipStream_t s, Timing *tm, const SideStream *side)
+                       hipStream_t s, Timing *tm, const SideStream *side, bool clear_empty)
 {
     Timing off;
     if (tm == nullptr) tm = &off;
@@ -701,13 +704,14 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
     // Empty tiles depend only on the histogram: clear them on the side stream
     // while the scan, scatter and tile kernels run (disjoint cells).
     hipStream_t zs = s;
-    if (!accumulate && side && side->s) {
+    const bool zero_empty = !accumulate && clear_empty;
+    if (zero_empty && side && side->s) {
         hipError_t e = hipEventRecord(side->fork, s);
         if (e == hipSuccess) e = hipStreamWaitEvent(side->s, side->fork, 0);
         if (e != hipSuccess) return e;
         zs = side->s;
     }
-    if (!accumulate) {
+    if (zero_empty) {
         const uint32_t zb = min(g.nb, 8192u);
         tm->begin(RICK_STAGE_ZERO, zs);
         if (precision == RICK_FP64)

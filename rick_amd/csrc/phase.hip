@@ -16,6 +16,7 @@
 // reduction to [-pi, pi] stay FP64, the rotation and accumulation of the
 // plane terms use FP32 sin/cos and an FP64 accumulator.
 #include "rick_internal.h"
+#include "rick_math.h"
 
 #pragma clang fp contract(off)
 
@@ -33,53 +34,26 @@ k_phase(PhaseGeom g, const void *__restrict__ gridss_v, double *__restrict__ ore
     const int iv = blockIdx.y;
     if (iu >= g.nx) return;
     const size_t pix = (size_t)iv * g.nx + iu;
-    // Physical row of logical row iv in the plane stack.
-    const int prow = g.row_n1 ? (g.ny / g.row_n1) * (iv % g.row_n1) + iv / g.row_n1 : iv;
-    const size_t src = (size_t)prow * g.nx + iu;
     const size_t plane = (size_t)g.nx * g.ny;
     double acc_re = 0.0, acc_im = 0.0;
     const bool phase = g.phase_on && g.planes > 1;
-    double n = 1.0;
-    if (phase) {
-        const double l = g.ltab[iu], m = g.mtab[iv];
-        const double radius2 = (l * l + m * m);
-        n = sqrt(1 - radius2);
-    }
+    const double n = phase ? pixel_n(g.ltab[iu], g.mtab[iv]) : 1.0;
     for (int iw = 0; iw < g.planes; ++iw) {
-        const size_t idx = src + (size_t)iw * plane;
-        double p, q;
+        const size_t idx = pix + (size_t)iw * plane;
         if constexpr (PREC == RICK_FP64) {
             const double2 v = reinterpret_cast<const double2 *>(gridss_v)[idx];
-            p = v.x;
-            q = v.y;
+            if (phase) phase_add_f64(v.x, v.y, g.wmin + iw * g.dw, n, g.dwnorm, acc_re, acc_im);
+            else {
+                acc_re += v.x;
+                acc_im += v.y;
+            }
         } else {
             const float2 v = reinterpret_cast<const float2 *>(gridss_v)[idx];
-            p = v.x;
-            q = v.y;
-        }
-        if (!phase) {
-            acc_re += p;
-            acc_im += q;
-            continue;
-        }
-        // phase_correction.cu:200, 222-223, 233-235.
-        const double wterm = g.wmin + iw * g.dw;
-        const double arg = 2.0 * kRefPi * wterm * (n - 1.0);
-        if constexpr (PREC == RICK_FP64) {
-            double s, c;
-            sincos(arg, &s, &c);
-            acc_re += (p * c - q * s) * g.dwnorm * n;
-            acc_im += (p * s + q * c) * g.dwnorm * n;
-        } else {
-            // Reduce in FP64, rotate in FP32.
-            const double k = rint(arg * (1.0 / (2.0 * 3.141592653589793)));
-            const float red = (float)(arg - k * (2.0 * 3.141592653589793));
-            float s, c;
-            sincosf(red, &s, &c);
-            const float pf = (float)p, qf = (float)q;
-            const float fre = (pf * c - qf * s), fim = (pf * s + qf * c);
-            acc_re += (double)fre * g.dwnorm * n;
-            acc_im += (double)fim * g.dwnorm * n;
+            if (phase) phase_add_f32(v.x, v.y, g.wmin + iw * g.dw, n, g.dwnorm, acc_re, acc_im);
+            else {
+                acc_re += (double)v.x;
+                acc_im += (double)v.y;
+            }
         }
     }
     if (accumulate) {

@@ -85,20 +85,19 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
                        const double *table_d, uint32_t num_points, uint32_t F, uint32_t P,
                        const double *uu, const double *vv, const double *ww,
                        const float *vr, const float *vi, const float *wt, void *grid,
-                       int accumulate, hipStream_t s, Timing *tm, const SideStream *side);
+                       int accumulate, hipStream_t s, Timing *tm, const SideStream *side,
+                       bool clear_empty = true);
 
 // FFT: per-plane 2-D backward transform of planes x ny x nx complex, scaled.
 struct FftPlan {
     int planes, nx, ny;
     const void *tw_x, *tw_y;   // device twiddle tables exp(+2 pi i m/N), N entries
 };
-// Long columns (ny above 512) use two passes.  An in-place transform either
-// stages each plane through tmp (one plane; natural row order), or, when
-// row_n1 is given, runs fully in place and leaves logical row v at physical
-// row (ny/N1)*(v % N1) + v / N1, reporting N1 in *row_n1 (0: natural order).
+// Long columns (ny above 512) use two passes; an in-place transform stages
+// each plane through tmp (one plane).  Output in natural row order.
 bool fft_needs_tmp(int precision, int ny);
 hipError_t launch_fft(const FftPlan &p, int precision, const void *in, void *out, void *tmp,
-                      double scale, hipStream_t s, Timing *tm, int *row_n1);
+                      double scale, hipStream_t s, Timing *tm);
 
 // Phase correction + plane sum.
 struct PhaseGeom {
@@ -106,9 +105,18 @@ struct PhaseGeom {
     double wmin, dw, dwnorm;
     int phase_on;
     const double *ltab, *mtab;  // device: sin(xcoord*resolution) per u / per v index
-    int row_n1;                 // 0, or the FFT's row permutation (see launch_fft)
 };
 hipError_t launch_phase(const PhaseGeom &p, int precision, const void *gridss, double *re,
                         double *im, int accumulate, hipStream_t s, Timing *tm);
+
+// Image chain after gridding, in place on grid (clobbered) and without
+// scratch: row FFT reading only the tiles whose gridder bucket count
+// (occ_counts, geometry occ_geom; null: dense grid) is non-zero, column FFT
+// whose last pass is fused with the phase correction and plane sum into the
+// images (overwritten).  Scales by `scale` (1/(Nx*Ny)).
+hipError_t launch_image_chain(const FftPlan &p, int precision, void *grid, double scale,
+                              hipStream_t s, Timing *tm, const GridGeom *occ_geom,
+                              const uint32_t *occ_counts, const PhaseGeom &pg, double *re,
+                              double *im);
 
 }  // namespace rick

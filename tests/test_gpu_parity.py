@@ -242,11 +242,33 @@ def test_chain_fourstep_matches_oracle(precision):
     ins = [dev(a) for a in (ds.uu, ds.vv, ds.ww, ds.vis_real, ds.vis_img, ds.weight)]
     gre, gim, _ = plan.image(*ins, cfg.n_chan, cfg.n_pol)
     err = rel_l2(host(gre) + 1j * host(gim), ref)
-    # The natural-order FFT entry point on the same grid agrees too.
+    # The natural-order FFT entry point on the same grid agrees too, and so
+    # does the unfused grid -> fft -> phase sequence.
     g = plan.grid(*ins, cfg.n_chan, cfg.n_pol)
     out = torch.empty_like(g)
     plan.fft(g, out)
     e_fft = rel_l2(host(out), gridss)
-    print(f"fourstep precision={precision}: image rel-L2 {err:.3e}, fft rel-L2 {e_fft:.3e}")
+    sre, sim = plan.phase(out)
+    e_sep = rel_l2(host(sre) + 1j * host(sim), ref)
+    e_cs = rel_l2(host(gre) + 1j * host(gim), host(sre) + 1j * host(sim))
+    print(f"fourstep precision={precision}: image rel-L2 {err:.3e}, fft rel-L2 {e_fft:.3e}, "
+          f"separate-path image {e_sep:.3e}, chain vs separate {e_cs:.3e}")
     tol = FP64_TOL if precision == FP64 else FP32_TOL
-    assert err <= tol and e_fft <= tol
+    assert err <= tol and e_fft <= tol and e_sep <= tol
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32])
+def test_phase_large_w_matches_oracle(precision):
+    """|w| up to 9e4 wavelengths: phase arguments of ~1e3 rad exercise the
+    argument reduction of the device sin/cos."""
+    nx, ny, planes = 64, 48, 8
+    rng = np.random.default_rng(17)
+    gridss = rng.normal(size=2 * planes * nx * ny)
+    res, wmin, wmax = 1.0 / 900.0, -3.0e4, 9.0e4
+    re, im = O.phase_correction(gridss, nx, ny, planes, res, wmin, wmax, threads=4)
+    plan = Plan(nx, ny, planes, resolution=res, wmin=wmin, wmax=wmax, precision=precision)
+    src = dev(gridss if precision == FP64 else gridss.astype(np.float32))
+    gre, gim = plan.phase(src)
+    err = rel_l2(host(gre) + 1j * host(gim), re + 1j * im)
+    print(f"large-w phase precision={precision}: rel-L2 {err:.3e}")
+    assert err <= (1e-13 if precision == FP64 else 1e-6)
