@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--rows-per-task", type=int, default=0)
     ap.add_argument("--tune", action="store_true",
                     help="autotune tile shape and rows/task first (rick_amd.autotune)")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the host-pointer wstack() drop-in timing (PCIe-inclusive)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one 1 GiB device-to-device copy (known HBM bytes) so that "
                          "profiler byte counters can be calibrated against it")
@@ -137,6 +139,28 @@ def cpu_baseline(ds_host_fn, cfg, target_s):
                       f"(KAISERBESSEL, support {cfg.support}) into the full {nx}^2 x "
                       f"{cfg.planes}-plane FP64 grid with {threads} OpenMP threads; "
                       f"{t:.2f} s", "cpu": _cpu_model()}
+
+
+def dropin_rate(ds, cfg, torch):
+    """The host-pointer drop-in wstack() on the full workload: inputs copied
+    to the GPU, gridded, and the device grid added into the caller's host
+    grid (FP64).  PCIe-inclusive; reported beside ``value``, never as it."""
+    from rick_amd import imaging
+    h = ds.to_numpy()
+    nx = cfg.grid
+    grid = np.zeros(2 * cfg.planes * nx * nx, dtype=np.float64)
+    args = (cfg.planes, cfg.n_rows, cfg.n_chan, cfg.n_pol, h.uu, h.vv, h.ww, h.vis_real,
+            h.vis_img, h.weight, 1.0 / nx, 1.0 / cfg.planes, cfg.support, nx, nx, grid)
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    imaging.set_kernel(cfg.kernel)
+    imaging.wstack(*args, num_threads=threads)          # warm-up: plan + pinned buffers
+    t0 = time.perf_counter()
+    imaging.wstack(*args, num_threads=threads)
+    t = time.perf_counter() - t0
+    del grid, h
+    return {"vis_per_s": cfg.n_vis / t, "s_per_call": t, "host_threads": threads,
+            "what": "wstack() on host arrays: H2D of uvw/vis/weights, FP64 gridding, D2H of "
+                    "the FP64 grid through pinned chunks, += into the host grid"}
 
 
 def _cpu_model():
@@ -260,6 +284,9 @@ def main():
             "image_rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
             "image_peak_rel": float(np.abs(a - b).max() / np.abs(b).max()),
         }
+    if not args.no_dropin and not args.only:
+        log("drop-in wstack() on host arrays")
+        out["dropin_pcie_inclusive"] = dropin_rate(ds, cfg, torch)
     if not args.no_cpu:
         log("cpu baseline")
         mid = cfg.n_rows // 2

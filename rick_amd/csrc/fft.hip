@@ -389,7 +389,7 @@ struct PhaseSink {
 };
 
 template <typename T, int L>
-__global__ void __launch_bounds__(256, 2)
+__global__ void __launch_bounds__(256)
 k_fft_cols_phase(ColArgs a, int planes, const C2<T> *__restrict__ tw,
                  const C2<T> *__restrict__ grid, PhaseGeom pg, int vs_mul, int v_stride,
                  double *__restrict__ ore, double *__restrict__ oim)
