@@ -88,21 +88,28 @@ def run_path(ds, ins, precision, steps, warmup, torch, tile=(0, 0), rpt=0):
         plan.image(*ins, F, P, grid=grid, image_real=re, image_imag=im)
 
     for _ in range(warmup):
+        grid_step()
         image_step()
     grid_ms = timed(grid_step, steps, torch)
     image_ms = timed(image_step, steps, torch)
-    # Per-kernel durations (HIP events on the launch stream).
-    plan.profile(True)
-    for _ in range(steps):
-        image_step()
-    st = plan.stage_times()
-    plan.profile(False)
+
+    # Per-kernel durations (HIP events on the launch stream), for the
+    # gridding step (dense grid) and for the image chain (sparse grid).
+    def stage_profile(fn):
+        plan.profile(True)
+        for _ in range(steps):
+            fn()
+        st = plan.stage_times()
+        plan.profile(False)
+        return {k: {"ms_per_launch": (ms / n if n else 0.0), "launches_per_step": n / steps,
+                    "ms_per_step": ms / steps}
+                for k, (ms, n) in st.items() if n}
+
+    stages = stage_profile(grid_step)
     stats = plan.grid_stats()
-    stages = {k: {"ms_per_launch": (ms / n if n else 0.0), "launches_per_step": n / steps,
-                  "ms_per_step": ms / steps}
-              for k, (ms, n) in st.items() if n}
+    image_stages = stage_profile(image_step)
     img = (re.cpu().numpy() + 1j * im.cpu().numpy())
-    return plan, grid_ms, image_ms, stages, stats, img
+    return plan, grid_ms, image_ms, stages, image_stages, stats, img
 
 
 def cpu_baseline(ds_host_fn, cfg, target_s):
@@ -211,10 +218,11 @@ def main():
             if tuned:
                 tile, rpt = tuned
         log(f"{name}: tile {tile} rows/task {rpt}; warmup {args.warmup}, steps {args.steps}")
-        plan, gms, ims, stages, stats, img = run_path(ds, ins, prec, args.steps, args.warmup,
-                                                      torch, tile, rpt)
+        plan, gms, ims, stages, istages, stats, img = run_path(ds, ins, prec, args.steps,
+                                                               args.warmup, torch, tile, rpt)
         res[name] = {"grid_ms": gms, "image_ms": ims, "grid_vis_per_s": vis / (gms * 1e-3),
-                     "stages": stages, "grid_stats": stats, "img": img,
+                     "stages": stages, "image_stages": istages, "grid_stats": stats,
+                     "img": img,
                      "grid_bytes": plan.grid_bytes,
                      "tile": [plan.desc.tile_u or 64, plan.desc.tile_v or 64],
                      "rows_per_task": plan.desc.rows_per_task or 4096}
@@ -228,14 +236,13 @@ def main():
     cb = 8 if main_path == "fp32" else 16
     # Dominant gridding kernel and its algorithmic bytes.
     grid_stages = {k: v for k, v in r["stages"].items()
-                   if k in ("count", "scan", "scatter", "zero_empty", "zero_split", "tile")}
+                   if k in ("count", "scan", "scatter", "zero_split", "tile")}
     dom = max(grid_stages, key=lambda k: grid_stages[k]["ms_per_step"])
     grid_bytes = cfg.planes * cfg.grid * cfg.grid * cb
-    tile_bytes = r["tile"][0] * r["tile"][1] * cb
-    nonempty = r["grid_stats"]["nonempty_buckets"]
-    alg = {"tile": nonempty * tile_bytes,                   # cells of non-empty tiles, once
-           "zero_empty": grid_bytes - nonempty * tile_bytes,  # cells of empty tiles, once
-           "scatter": rows * (24 + 4 * P) + vis * 8,        # uvw, weights, visibilities read once
+    rec_bytes = 32 if main_path == "fp64" else 24
+    records = r["grid_stats"]["records"]
+    alg = {"tile": grid_bytes + records * rec_bytes,        # every cell written once, records read
+           "scatter": rows * (24 + 4 * P) + vis * 8 + records * rec_bytes,  # inputs read, records written
            "count": rows * 24}.get(dom)
     t_dom = grid_stages[dom]["ms_per_launch"] * 1e-3
     achieved = alg / t_dom / 1e9 if alg else None
@@ -275,7 +282,9 @@ def main():
                      "grid_vis_per_s": v["grid_vis_per_s"], "grid_stats": v["grid_stats"],
                      "tile": v["tile"], "rows_per_task": v["rows_per_task"],
                      "stages_ms_per_step": {k: round(s["ms_per_step"], 4)
-                                            for k, s in v["stages"].items()}}
+                                            for k, s in v["stages"].items()},
+                     "image_stages_ms_per_step": {k: round(s["ms_per_step"], 4)
+                                                  for k, s in v["image_stages"].items()}}
     if "fp32" in res and "fp64" in res:
         a, b = res["fp32"]["img"], res["fp64"]["img"]
         out["fp32_vs_fp64"] = {

@@ -151,9 +151,9 @@ enum {
     RICK_STAGE_COUNT_ROWS = 0, /* bucket histogram (incl. clearing it)        */
     RICK_STAGE_SCAN = 1,       /* bucket / task offsets                        */
     RICK_STAGE_SCATTER = 2,    /* per-row channel sums -> sorted records       */
-    RICK_STAGE_ZERO = 3,       /* clear tiles of empty buckets (side stream,   */
-                               /* concurrent with stages 1, 2, 4, 9)          */
-    RICK_STAGE_TILE = 4,       /* LDS-tile accumulation and grid write         */
+    RICK_STAGE_ZERO = 3,       /* reserved: empty tiles are cleared by zero    */
+                               /* tasks inside stage 4                         */
+    RICK_STAGE_TILE = 4,       /* LDS-tile accumulation, grid write, clearing  */
     RICK_STAGE_FFT_COLS = 5,   /* column FFT (single pass, or four-step A)     */
     RICK_STAGE_FFT_COLS_B = 6, /* column FFT four-step B                       */
     RICK_STAGE_FFT_ROWS = 7,   /* row FFT (+ 1/(Nx*Ny))                         */

@@ -13,7 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / "librick_hip.so"
 RICK_OK, RICK_ERR_INVALID, RICK_ERR_UNSUPPORTED, RICK_ERR_DEVICE, RICK_ERR_NOMEM = range(5)
 GAUSS, KAISERBESSEL, GAUSS_HI_PRECISION = 0, 1, 2
 FP64, FP32 = 0, 1
-STAGES = ("count", "scan", "scatter", "zero_empty", "tile", "fft_cols", "fft_cols_b",
+STAGES = ("count", "scan", "scatter", "reserved", "tile", "fft_cols", "fft_cols_b",
           "fft_rows", "phase", "zero_split")
 
 _c_int, _c_uint, _c_double, _c_size, _vp = (ctypes.c_int, ctypes.c_uint, ctypes.c_double,

@@ -73,7 +73,6 @@ struct rick_plan_s {
     bool fft_ok = false;
     bool grid_ok = false;
     Timing timing;
-    SideStream side;
 };
 
 extern "C" {
@@ -118,9 +117,6 @@ int rick_plan_destroy(rick_plan p)
             hipEventDestroy(e.first);
             hipEventDestroy(e.second);
         }
-    if (p->side.s) hipStreamDestroy(p->side.s);
-    if (p->side.fork) hipEventDestroy(p->side.fork);
-    if (p->side.join) hipEventDestroy(p->side.join);
     hipSetDevice(dev);
     delete p;
     return RICK_OK;
@@ -206,17 +202,7 @@ int rick_plan_create(rick_plan *out, const rick_plan_desc *desc)
     char *base = static_cast<char *>(p->dmem);
     p->table_d = reinterpret_cast<double *>(base + o_table);
     if (o_tmp != o_work) p->tmp_plane = base + o_tmp;
-    if (p->grid_ok) {
-        grid_work_carve(g, d.max_points, d.precision, base + o_work, &p->work);
-        // Side stream at the default priority, non-blocking w.r.t. the null stream.
-        if (hipStreamCreateWithFlags(&p->side.s, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&p->side.fork, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&p->side.join, hipEventDisableTiming) != hipSuccess) {
-            hipFree(p->dmem);
-            hipSetDevice(prev);
-            return set_error(RICK_ERR_DEVICE, "cannot create the plan's side stream");
-        }
-    }
+    if (p->grid_ok) grid_work_carve(g, d.max_points, d.precision, base + o_work, &p->work);
 
     // Host tables -> device (synchronous; plan creation is not on a hot path).
     std::vector<double> h(kMaxTable, 0.0);
@@ -288,7 +274,7 @@ int rick_plan_grid(rick_plan p, unsigned int n, unsigned int F, unsigned int P, 
     RICK_HIP(hipSetDevice(p->d.device));
     RICK_HIP(launch_grid(p->geom, p->work, p->d.precision, p->table_d, n, F, P, uu, vv, ww, vr,
                          vi, wt, grid, accumulate, static_cast<hipStream_t>(stream),
-                         &p->timing, &p->side));
+                         &p->timing));
     return RICK_OK;
 }
 
@@ -397,11 +383,11 @@ int rick_plan_image(rick_plan p, unsigned int n, unsigned int F, unsigned int P,
     // such grids are cleared densely instead.)
     const bool sparse = p->geom.nx / p->geom.tu <= 256;
     RICK_HIP(launch_grid(p->geom, p->work, p->d.precision, p->table_d, n, F, P, uu, vv, ww, vr,
-                         vi, wt, grid, 0, s, &p->timing, &p->side, !sparse));
+                         vi, wt, grid, 0, s, &p->timing, !sparse));
     const double norm = 1.0 / (double)((double)p->geom.nx * (double)p->geom.ny);
     RICK_HIP(launch_image_chain(p->fft, p->d.precision, grid, norm, s, &p->timing,
                                 sparse ? &p->geom : nullptr, sparse ? p->work.counts : nullptr,
-                                p->phase, re, im));
+                                sparse ? p->work.band : nullptr, p->phase, re, im));
     return RICK_OK;
 }
 

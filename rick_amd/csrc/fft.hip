@@ -187,6 +187,11 @@ struct Lines {
     // is taken as zero (cells of empty gridder tiles are never written).
     const unsigned char *occ;
     int occ_stride, occ_tu;
+    // Optional row-band occupancy of the input (columns): element n is read
+    // only if rowlive[(rl_row0 + n * rl_rstride) >> rl_shift] is non-zero, else
+    // it is zero (rows of empty bands are skipped by the row transform).
+    const unsigned char *rowlive;
+    int rl_row0, rl_rstride, rl_shift;
 };
 
 // Butterfly b of a pass with L/R butterflies per line: rows map consecutive
@@ -240,8 +245,11 @@ __device__ __forceinline__ void pass(C2<T> *vals, int tid, const Lines &ln, int 
                 if constexpr (from_lds) {
                     v[r] = ROWS ? lds[g * LP + lpad(n)] : lds[n * G + g];
                 } else {
-                    const bool live = g < ln.nvalid &&
-                                      (ln.occ == nullptr || ln.occ[g * ln.occ_stride + n / ln.occ_tu]);
+                    const bool live =
+                        g < ln.nvalid &&
+                        (ln.occ == nullptr || ln.occ[g * ln.occ_stride + n / ln.occ_tu]) &&
+                        (ln.rowlive == nullptr ||
+                         ln.rowlive[(ln.rl_row0 + n * ln.rl_rstride) >> ln.rl_shift]);
                     v[r] = live ? gin[ln.in_base + (size_t)g * ln.in_line + (size_t)n * ln.in_el]
                                 : C2<T>{T(0), T(0)};
                 }
@@ -284,10 +292,46 @@ constexpr int oc_bytes(int L) { return lines_per_wg(L) * (L / 32 > 0 ? L / 32 : 
 
 // Tile occupancy of a sparse grid (gridder buckets): counts of the bucket
 // (plane, tile row, tile column); zero means the tile was never written.
+// band[plane * tiles_v + t] is non-zero iff tile row t of the plane holds an
+// occupied tile: the grid rows of an empty band are zero, so the row
+// transform skips them (writes nothing) and later column passes read them as
+// zero.
 struct Occupancy {
     const uint32_t *counts;
+    uint32_t *band;
     int tu, tv, tiles_u, tiles_v, ny;
 };
+
+__global__ void __launch_bounds__(256) k_band_occ(Occupancy oc, int planes)
+{
+    const int bi = blockIdx.x * 256 + threadIdx.x;
+    if (bi >= planes * oc.tiles_v) return;
+    const uint32_t *c = oc.counts + (size_t)bi * oc.tiles_u;
+    uint32_t any = 0;
+    for (int t = 0; t < oc.tiles_u; ++t) any |= c[t];
+    oc.band[bi] = any != 0;
+}
+
+// Row-band flags of one plane for the column kernels (null: dense input).
+struct ColOcc {
+    const uint32_t *band;
+    int tiles_v, tv_shift;
+};
+
+// Loads the plane's band flags into LDS and points the input at them.
+__device__ __forceinline__ void col_rowlive(const ColOcc &co, int plane, int row0, int rstride,
+                                            unsigned char *rl, Lines &ln)
+{
+    ln.rowlive = nullptr;
+    if (co.band == nullptr) return;
+    for (int t = threadIdx.x; t < co.tiles_v; t += blockDim.x)
+        rl[t] = co.band[(size_t)plane * co.tiles_v + t] != 0;
+    __syncthreads();
+    ln.rowlive = rl;
+    ln.rl_row0 = row0;
+    ln.rl_rstride = rstride;
+    ln.rl_shift = co.tv_shift;
+}
 
 // Rows: workgroup w handles rows w*G .. w*G+G-1 of the planes*ny rows.
 template <typename T, int L>
@@ -308,6 +352,18 @@ k_fft_rows(int rows_total, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T>
     ln.nvalid = (int)min((size_t)G, (size_t)rows_total - row0);
     ln.tw_set = 0;
     ln.occ = nullptr;
+    ln.rowlive = nullptr;
+    if (oc.band) {
+        // Uniform across the workgroup: all its rows in empty bands -> the
+        // output rows are zero and are neither computed nor written.
+        int live = 0;
+        for (int g = threadIdx.x; g < ln.nvalid; g += NT) {
+            const size_t row = row0 + g;
+            const int plane = (int)(row / oc.ny), v = (int)(row % oc.ny);
+            live |= oc.band[(size_t)plane * oc.tiles_v + v / oc.tv] != 0;
+        }
+        if (!__syncthreads_or(live)) return;
+    }
     if (oc.counts) {
         for (int i = threadIdx.x; i < G * oc.tiles_u && oc.tiles_u <= MAXT; i += NT) {
             const int g = i / oc.tiles_u, t = i % oc.tiles_u;
@@ -340,10 +396,12 @@ struct ColArgs {
 
 template <typename T, int L>
 __global__ void __launch_bounds__(L >= 8192 ? 512 : 256)
-k_fft_cols(ColArgs a, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gout, T scale)
+k_fft_cols(ColArgs a, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gout, T scale,
+           ColOcc co)
 {
     constexpr int G = lines_per_wg(L);
     __shared__ C2<T> lds[lds_elems(L, false)];
+    __shared__ unsigned char rl[256];
     const int c0 = blockIdx.x * G;
     const int s = blockIdx.y;
     const size_t plane = (size_t)blockIdx.z * (size_t)a.nx * (size_t)a.ny;
@@ -356,6 +414,7 @@ k_fft_cols(ColArgs a, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gou
     ln.nvalid = min(G, a.nx - c0);
     ln.tw_set = a.tw_by_set ? s : 0;
     ln.occ = nullptr;
+    col_rowlive(co, blockIdx.z, s * a.in_set_mul, a.in_stride, rl, ln);
     C2<T> vals[kE];
     GlobalSink<T> sink{gout, &ln, tw, scale};
     pass<T, L, G, false, 0>(vals, threadIdx.x, ln, a.ts, tw, lds, gin, sink);
@@ -392,11 +451,12 @@ template <typename T, int L>
 __global__ void __launch_bounds__(256)
 k_fft_cols_phase(ColArgs a, int planes, const C2<T> *__restrict__ tw,
                  const C2<T> *__restrict__ grid, PhaseGeom pg, int vs_mul, int v_stride,
-                 double *__restrict__ ore, double *__restrict__ oim)
+                 double *__restrict__ ore, double *__restrict__ oim, ColOcc co)
 {
     constexpr int G = lines_per_wg(L);
     constexpr int NT = G * L / kE;
     __shared__ C2<T> lds[lds_elems(L, false)];
+    __shared__ unsigned char rl[256];
     const int c0 = blockIdx.x * G;
     const int s = blockIdx.y;
     PhaseSink<T> sink;
@@ -418,8 +478,12 @@ k_fft_cols_phase(ColArgs a, int planes, const C2<T> *__restrict__ tw,
     ln.nvalid = sink.nvalid;
     ln.tw_set = 0;
     ln.occ = nullptr;
+    ln.rowlive = nullptr;
     for (int iw = 0; iw < planes; ++iw) {
         const size_t plane = (size_t)iw * (size_t)a.nx * (size_t)a.ny;
+        if (co.band) {
+            col_rowlive(co, iw, s * a.in_set_mul, a.in_stride, rl, ln);
+        }
         ln.in_base = plane + (size_t)s * a.in_set_mul * (size_t)a.nx + c0;
         ln.out_base = 0;
         sink.wterm = pg.wmin + iw * pg.dw;  // phase_correction.cu:200
@@ -469,13 +533,13 @@ hipError_t launch_rows(int L, int rows_total, const C2<T> *tw, const C2<T> *in, 
 
 template <typename T>
 hipError_t launch_cols(int L, const ColArgs &a, int sets, int planes, const C2<T> *tw,
-                       const C2<T> *in, C2<T> *out, hipStream_t s)
+                       const C2<T> *in, C2<T> *out, hipStream_t s, const ColOcc &co = ColOcc{})
 {
 #define RICK_COLS(LL)                                                                        \
     case LL: {                                                                               \
         constexpr int G = lines_per_wg(LL);                                                  \
         hipLaunchKernelGGL((k_fft_cols<T, LL>), dim3((a.nx + G - 1) / G, sets, planes),      \
-                           dim3(G * LL / kE), 0, s, a, tw, in, out, T(1));                   \
+                           dim3(G * LL / kE), 0, s, a, tw, in, out, T(1), co);               \
         return hipGetLastError();                                                            \
     }
     switch (L) {
@@ -489,14 +553,14 @@ hipError_t launch_cols(int L, const ColArgs &a, int sets, int planes, const C2<T
 template <typename T>
 hipError_t launch_cols_phase(int L, const ColArgs &a, int sets, int planes, const C2<T> *tw,
                              const C2<T> *grid, const PhaseGeom &pg, int vs_mul, int v_stride,
-                             double *re, double *im, hipStream_t s)
+                             double *re, double *im, hipStream_t s, const ColOcc &co)
 {
 #define RICK_CPH(LL)                                                                         \
     case LL: {                                                                               \
         constexpr int G = lines_per_wg(LL);                                                  \
         hipLaunchKernelGGL((k_fft_cols_phase<T, LL>), dim3((a.nx + G - 1) / G, sets, 1),     \
                            dim3(G * LL / kE), 0, s, a, planes, tw, grid, pg, vs_mul,         \
-                           v_stride, re, im);                                                \
+                           v_stride, re, im, co);                                            \
         return hipGetLastError();                                                            \
     }
     switch (L) {
@@ -564,9 +628,10 @@ hipError_t fft_impl(const FftPlan &p, const void *in_v, void *out_v, void *tmp_v
 }
 
 // Image chain: in place, rows first reading only occupied tiles of a sparse
-// grid, then (for ny > 512) four-step pass A in place with rows left in
-// transposed order, then the last column pass fused with the phase
-// correction and plane sum, writing the image.
+// grid and skipping rows of empty bands, then (for ny > 512) four-step pass A
+// in place with rows left in transposed order (empty bands read as zero),
+// then the last column pass fused with the phase correction and plane sum,
+// writing the image.
 template <typename T>
 hipError_t image_chain_impl(const FftPlan &p, void *grid_v, double scale_d, hipStream_t s,
                             Timing *tm, const Occupancy &oc, const PhaseGeom &pg, double *re,
@@ -577,6 +642,14 @@ hipError_t image_chain_impl(const FftPlan &p, void *grid_v, double scale_d, hipS
     const CT *twx = static_cast<const CT *>(p.tw_x);
     const CT *twy = static_cast<const CT *>(p.tw_y);
     const int nx = p.nx, ny = p.ny, planes = p.planes;
+    ColOcc co{};
+    if (oc.counts && oc.band) {
+        const int nbands = planes * oc.tiles_v;
+        hipLaunchKernelGGL(k_band_occ, dim3((nbands + 255) / 256), dim3(256), 0, s, oc, planes);
+        co.band = oc.band;
+        co.tiles_v = oc.tiles_v;
+        co.tv_shift = __builtin_ctz((unsigned)oc.tv);
+    }
     tm->begin(RICK_STAGE_FFT_ROWS, s);
     hipError_t e = launch_rows<T>(nx, planes * ny, twx, grid, grid, (T)scale_d, s, oc);
     tm->end(RICK_STAGE_FFT_ROWS, s);
@@ -584,7 +657,7 @@ hipError_t image_chain_impl(const FftPlan &p, void *grid_v, double scale_d, hipS
     if (!fft_needs_tmp(sizeof(T) == 8 ? RICK_FP64 : RICK_FP32, ny)) {
         const ColArgs a{nx, ny, 0, 1, 0, 1, 0, 1};
         tm->begin(RICK_STAGE_PHASE, s);
-        e = launch_cols_phase<T>(ny, a, 1, planes, twy, grid, pg, 0, 1, re, im, s);
+        e = launch_cols_phase<T>(ny, a, 1, planes, twy, grid, pg, 0, 1, re, im, s, co);
         tm->end(RICK_STAGE_PHASE, s);
         return e;
     }
@@ -594,11 +667,11 @@ hipError_t image_chain_impl(const FftPlan &p, void *grid_v, double scale_d, hipS
     // B: rows N2*k1 + n2 -> X[k1 + N1*k2], image row k1 + N1*k2.
     const ColArgs Bt{nx, ny, N2, 1, N2, 1, 0, N1};
     tm->begin(RICK_STAGE_FFT_COLS, s);
-    e = launch_cols<T>(N1, At, N2, planes, twy, grid, grid, s);
+    e = launch_cols<T>(N1, At, N2, planes, twy, grid, grid, s, co);
     tm->end(RICK_STAGE_FFT_COLS, s);
     if (e != hipSuccess) return e;
     tm->begin(RICK_STAGE_PHASE, s);
-    e = launch_cols_phase<T>(N2, Bt, N1, planes, twy, grid, pg, 1, N1, re, im, s);
+    e = launch_cols_phase<T>(N2, Bt, N1, planes, twy, grid, pg, 1, N1, re, im, s, ColOcc{});
     tm->end(RICK_STAGE_PHASE, s);
     return e;
 }
@@ -622,8 +695,8 @@ hipError_t launch_fft(const FftPlan &p, int precision, const void *in, void *out
 
 hipError_t launch_image_chain(const FftPlan &p, int precision, void *grid, double scale,
                               hipStream_t s, Timing *tm, const GridGeom *occ_geom,
-                              const uint32_t *occ_counts, const PhaseGeom &pg, double *re,
-                              double *im)
+                              const uint32_t *occ_counts, uint32_t *occ_band, const PhaseGeom &pg,
+                              double *re, double *im)
 {
     Timing off;
     if (tm == nullptr) tm = &off;
@@ -631,6 +704,7 @@ hipError_t launch_image_chain(const FftPlan &p, int precision, void *grid, doubl
     if (occ_geom) {
         if (!occ_counts || p.nx / occ_geom->tu > 256) return hipErrorInvalidValue;
         oc.counts = occ_counts;
+        oc.band = occ_band;
         oc.tu = occ_geom->tu;
         oc.tv = occ_geom->tv;
         oc.tiles_u = occ_geom->tiles_u;

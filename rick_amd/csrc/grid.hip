@@ -8,23 +8,28 @@
 //
 //   k_count      rows -> buckets (bucket = w-plane x TUxTV grid tile whose core
 //                the row's footprint touches; at most 4 per row).  Each
-//                workgroup aggregates its 1024 rows' keys in an LDS hash table
-//                and adds one count per distinct key to the global histogram,
-//                so hot central tiles see one global atomic per workgroup.
+//                workgroup aggregates the keys of its 8192 rows in an LDS hash
+//                table and adds one count per distinct key to the global
+//                histogram: device-scope atomics are slow on this chip (its
+//                eight L2 slices are not coherent with each other), and 8192
+//                consecutive rows touch only a few hundred distinct buckets.
 //   k_scan_*     multi-block exclusive scans of the histogram -> bucket
 //                offsets, the task list (a task is <= rmax records of one
-//                bucket) and the list of buckets split over several tasks.
+//                bucket, or a zero task), the list of split buckets.
 //   k_scatter    forms per row S = sum_{f,p} fl32(weight*vis) (NaN real parts
 //                skipped, w-stacking.cu:517) and writes a record {pos_u,
 //                pos_v, S} to each of its buckets; slots come from one
-//                returning global atomic per distinct key per workgroup.
-//   k_zero_*     clear the tiles of split and of empty buckets (overwrite
-//                mode); empty buckets get no task.
+//                returning global atomic per distinct key per 1024-row block.
+//   k_zero_split clears the tiles of split buckets (overwrite mode).
 //   k_tile       one workgroup per task: accumulates its records into an FP64
 //                LDS tile (one lane per record, LDS atomics), then writes
 //                the tile once with plain, coalesced stores; tasks of split
-//                buckets flush with HBM atomics instead.  Every grid cell is
-//                written by exactly one tile, so no memset of the grid is needed.
+//                buckets flush with HBM atomics instead.  In overwrite mode
+//                each empty bucket gets a zero task that clears its tile; zero
+//                tasks sit between the accumulation tasks in bucket order, so
+//                the bandwidth-bound clearing shares the launch with the
+//                LDS-bound accumulation.  Every grid cell is written exactly
+//                once, so no memset of the grid is needed.
 //
 // The per-cell weight K[jKer]*K[kKer] is bitwise the reference's; the per-cell
 // value S*c differs from the reference's sum_{f,p} fl(fl32(w*v)*c) only by
@@ -39,12 +44,22 @@ namespace rick {
 namespace {
 
 constexpr int kThreads = 256;                 // count / scatter / scan workgroups
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr int kScanBlock = 4 * kThreads;      // buckets per scan block
+// Scatter: 1024-row blocks whose distinct keys always fit the LDS hash table.
 constexpr int kRowsPerThread = 4;
 constexpr int kRowsPerBlock = kThreads * kRowsPerThread;
 constexpr int kHashBits = 12;
 constexpr int kHashSlots = 1 << kHashBits;    // >= distinct keys of one block (4 x 1024)
-constexpr uint32_t kEmpty = 0xFFFFFFFFu;
-constexpr int kScanBlock = 4 * kThreads;      // buckets per scan block
+// Count: 8192-row blocks, bounded probing, overflow -> direct global atomic.
+constexpr int kBinThreads = 512;
+constexpr int kBinRowsPerThread = 16;
+constexpr int kBinRows = kBinThreads * kBinRowsPerThread;
+constexpr int kBinHashBits = 11;
+constexpr int kBinSlots = 1 << kBinHashBits;
+constexpr int kMaxProbe = 32;
+constexpr uint32_t kZeroTask = 2u;            // task.w of a zero task
+constexpr int kZeroRun = 1;                   // empty buckets per zero task (at most)
 constexpr int kTileThreads = 512;
 constexpr int kPadU = 8;  // LDS row padding (elements): rows shift by 8 banks
 
@@ -117,6 +132,20 @@ __device__ __forceinline__ int row_keys(const GridGeom &g, const Foot &f, uint32
     return n;
 }
 
+// LDS open-addressing hash set of 2^HB bucket keys with at most kMaxProbe
+// probes: returns the key's slot, or kEmpty when none was free on its path.
+template <int HB>
+__device__ __forceinline__ uint32_t hinsert_bounded(uint32_t *hkey, uint32_t key)
+{
+    uint32_t h = (key * 2654435761u) >> (32 - HB);
+    for (int p = 0; p < kMaxProbe; ++p) {
+        const uint32_t old = atomicCAS(&hkey[h], kEmpty, key);
+        if (old == kEmpty || old == key) return h;
+        h = (h + 1) & ((1u << HB) - 1u);
+    }
+    return kEmpty;
+}
+
 // LDS open-addressing hash set of bucket keys.
 __device__ __forceinline__ uint32_t hinsert(uint32_t *hkey, uint32_t key)
 {
@@ -128,35 +157,39 @@ __device__ __forceinline__ uint32_t hinsert(uint32_t *hkey, uint32_t key)
     }
 }
 
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         p[i] = v;
 }
 
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kBinThreads)
 k_count(GridGeom g, uint32_t n, const double *__restrict__ uu, const double *__restrict__ vv,
         const double *__restrict__ ww, uint32_t *__restrict__ counts)
 {
-    __shared__ uint32_t hkey[kHashSlots], hcnt[kHashSlots];
-    for (int i = threadIdx.x; i < kHashSlots; i += kThreads) {
+    __shared__ uint32_t hkey[kBinSlots], hcnt[kBinSlots];
+    for (int i = threadIdx.x; i < kBinSlots; i += kBinThreads) {
         hkey[i] = kEmpty;
         hcnt[i] = 0;
     }
     __syncthreads();
-    const uint32_t base = blockIdx.x * kRowsPerBlock;
-#pragma unroll
-    for (int k = 0; k < kRowsPerThread; ++k) {
-        const uint32_t i = base + k * kThreads + threadIdx.x;
+    const uint32_t base = blockIdx.x * (uint32_t)kBinRows;
+    for (int k = 0; k < kBinRowsPerThread; ++k) {
+        const uint32_t i = base + k * kBinThreads + threadIdx.x;
         Foot f;
         if (i < n && footprint(g, uu[i] / g.dx, vv[i] / g.dx, ww[i] / g.dw, f)) {
             uint32_t keys[4];
             const int nk = row_keys(g, f, keys);
-            for (int q = 0; q < nk; ++q) atomicAdd(&hcnt[hinsert(hkey, keys[q])], 1u);
+            for (int q = 0; q < nk; ++q) {
+                const uint32_t h = hinsert_bounded<kBinHashBits>(hkey, keys[q]);
+                if (h == kEmpty) atomicAdd(&counts[keys[q]], 1u);
+                else atomicAdd(&hcnt[h], 1u);
+            }
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kHashSlots; i += kThreads) {
+    for (int i = threadIdx.x; i < kBinSlots; i += kBinThreads) {
         const uint32_t c = hcnt[i];
         if (c) atomicAdd(&counts[hkey[i]], c);
     }
@@ -164,30 +197,50 @@ k_count(GridGeom g, uint32_t n, const double *__restrict__ uu, const double *__r
 
 // Scan of the histogram in three kernels: per-block sums, a single-block scan
 // of those, and the per-block downsweep that emits offsets and tasks.
-// Tasks of a bucket with c records (empty buckets have none: their tiles are
-// cleared by k_zero_empty).
+// Tasks of a bucket with c > 0 records.
 __device__ __forceinline__ uint32_t ntasks_of(uint32_t c, uint32_t rmax)
 {
     return (c + rmax - 1) / rmax;
 }
 
-__global__ void __launch_bounds__(kThreads)
-k_scan_reduce(GridGeom g, const uint32_t *__restrict__ counts, uint32_t *__restrict__ blocksum,
-              uint32_t nscan)
+// Tasks of a scan thread's 4 consecutive buckets (nv of them exist): those of
+// the non-empty ones, plus, when the grid is overwritten (zero != 0), one zero
+// task per kZeroRun consecutive empty ones.
+__device__ __forceinline__ uint32_t group_tasks(const uint32_t *c, int nv, uint32_t rmax,
+                                                int zero)
 {
-    __shared__ uint32_t s[4][kThreads];
-    uint32_t r = 0, t = 0, sp = 0, ne = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t b = blockIdx.x * kScanBlock + threadIdx.x * 4 + k;
-        if (b < g.nb) {
-            const uint32_t c = counts[b], nt = ntasks_of(c, g.rmax);
-            r += c;
-            t += nt;
-            sp += nt > 1 ? 1u : 0u;
-            ne += c > 0 ? 1u : 0u;
+    uint32_t t = 0;
+    int run = 0;  // length of the current run of empty buckets
+    for (int k = 0; k < nv; ++k) {
+        if (c[k]) {
+            t += ntasks_of(c[k], rmax);
+            run = 0;
+        } else {
+            if (zero && run % kZeroRun == 0) ++t;
+            ++run;
         }
     }
+    return t;
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_scan_reduce(GridGeom g, const uint32_t *__restrict__ counts, uint32_t *__restrict__ blocksum,
+              uint32_t nscan, int zero)
+{
+    __shared__ uint32_t s[4][kThreads];
+    uint32_t r = 0, sp = 0, ne = 0, c[4] = {0, 0, 0, 0};
+    const uint32_t b0 = blockIdx.x * kScanBlock + threadIdx.x * 4;
+    const int nv = b0 >= g.nb ? 0 : (int)min(4u, g.nb - b0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k < nv) {
+            c[k] = counts[b0 + k];
+            r += c[k];
+            sp += c[k] > g.rmax ? 1u : 0u;
+            ne += c[k] > 0 ? 1u : 0u;
+        }
+    }
+    const uint32_t t = group_tasks(c, nv, g.rmax, zero);
     s[0][threadIdx.x] = r;
     s[1][threadIdx.x] = t;
     s[2][threadIdx.x] = sp;
@@ -250,24 +303,24 @@ k_scan_top(GridGeom g, uint32_t *__restrict__ blocksum, uint32_t nscan,
 }
 
 __global__ void __launch_bounds__(kThreads)
-k_scan_down(GridGeom g, const uint32_t *__restrict__ counts, const uint32_t *__restrict__ blockoff,
-            uint32_t nscan, uint32_t *__restrict__ offsets, uint32_t *__restrict__ cursor,
-            uint4 *__restrict__ tasks, uint32_t task_cap, uint32_t *__restrict__ split_list,
-            uint32_t split_cap)
+k_scan_down(GridGeom g, const uint32_t *__restrict__ counts,
+            const uint32_t *__restrict__ blockoff, uint32_t nscan, uint32_t *__restrict__ offsets,
+            uint32_t *__restrict__ cursor, uint4 *__restrict__ tasks, uint32_t task_cap,
+            uint32_t *__restrict__ split_list, uint32_t split_cap, int zero)
 {
     __shared__ uint32_t s[3][kThreads];
-    uint32_t c[4], v[3] = {0, 0, 0};
+    uint32_t c[4] = {0, 0, 0, 0}, v[3] = {0, 0, 0};
     const uint32_t b0 = blockIdx.x * kScanBlock + threadIdx.x * 4;
+    const int nv = b0 >= g.nb ? 0 : (int)min(4u, g.nb - b0);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        c[k] = b0 + k < g.nb ? counts[b0 + k] : 0u;
-        const uint32_t nt = ntasks_of(c[k], g.rmax);
-        if (b0 + k < g.nb) {
+        if (k < nv) {
+            c[k] = counts[b0 + k];
             v[0] += c[k];
-            v[1] += nt;
-            v[2] += nt > 1 ? 1u : 0u;
+            v[2] += c[k] > g.rmax ? 1u : 0u;
         }
     }
+    v[1] = group_tasks(c, nv, g.rmax, zero);
 #pragma unroll
     for (int a = 0; a < 3; ++a) s[a][threadIdx.x] = v[a];
     __syncthreads();
@@ -284,20 +337,31 @@ k_scan_down(GridGeom g, const uint32_t *__restrict__ counts, const uint32_t *__r
     uint32_t ro = blockoff[blockIdx.x] + s[0][threadIdx.x] - v[0];
     uint32_t to = blockoff[nscan + blockIdx.x] + s[1][threadIdx.x] - v[1];
     uint32_t so = blockoff[2 * nscan + blockIdx.x] + s[2][threadIdx.x] - v[2];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    int run = 0;
+    for (int k = 0; k < nv; ++k) {
         const uint32_t b = b0 + k;
-        if (b >= g.nb) break;
         offsets[b] = ro;
         cursor[b] = ro;
-        const uint32_t nt = ntasks_of(c[k], g.rmax);
-        for (uint32_t ch = 0; ch < nt; ++ch) {
-            const uint32_t r0 = ro + ch * g.rmax, r1 = min(r0 + g.rmax, ro + c[k]);
-            if (to + ch < task_cap) tasks[to + ch] = make_uint4(b, r0, r1, nt > 1 ? 1u : 0u);
+        if (c[k] == 0) {
+            // Every kZeroRun-th bucket of a run of empty ones starts a zero task.
+            if (zero && run % kZeroRun == 0) {
+                uint32_t len = 1;
+                while (len < (uint32_t)kZeroRun && k + (int)len < nv && c[k + len] == 0) ++len;
+                if (to < task_cap) tasks[to] = make_uint4(b, len, 0u, kZeroTask);
+                ++to;
+            }
+            ++run;
+        } else {
+            run = 0;
+            const uint32_t nt = ntasks_of(c[k], g.rmax);
+            for (uint32_t q = 0; q < nt; ++q) {
+                const uint32_t r0 = ro + q * g.rmax, r1 = min(r0 + g.rmax, ro + c[k]);
+                if (to + q < task_cap) tasks[to + q] = make_uint4(b, r0, r1, nt > 1 ? 1u : 0u);
+            }
+            if (nt > 1 && so < split_cap) split_list[so++] = b;
+            to += nt;
         }
-        if (nt > 1 && so < split_cap) split_list[so++] = b;
         ro += c[k];
-        to += nt;
     }
 }
 
@@ -434,29 +498,6 @@ k_zero_split(GridGeom g, const uint32_t *__restrict__ split_list, const uint32_t
     }
 }
 
-// Clears the tiles of empty buckets (overwrite mode): every cell of the grid
-// is written either here or by exactly one tile task.
-template <typename T2>
-__global__ void __launch_bounds__(kThreads)
-k_zero_empty(GridGeom g, const uint32_t *__restrict__ counts, T2 *__restrict__ grid)
-{
-    const uint32_t per_plane = (uint32_t)g.tiles_u * (uint32_t)g.tiles_v;
-    for (uint32_t b = blockIdx.x; b < g.nb; b += gridDim.x) {
-        if (counts[b] != 0) continue;
-        const int w = (int)(b / per_plane), rem = (int)(b % per_plane);
-        const int tv = rem / g.tiles_u, tu = rem % g.tiles_u;
-        const int j0 = tu * g.tu, k0 = tv * g.tv;
-        const int nj = min(g.tu, g.nx - j0), nk = min(g.tv, g.ny - k0);
-        T2 z;
-        z.x = 0;
-        z.y = 0;
-        for (int idx = threadIdx.x; idx < nk * g.tu; idx += kThreads) {
-            const int kk = idx / g.tu, jj = idx % g.tu;
-            if (jj >= nj) continue;
-            grid[((size_t)w * g.ny + (size_t)(k0 + kk)) * (size_t)g.nx + (size_t)(j0 + jj)] = z;
-        }
-    }
-}
 
 // Per-task LDS tile accumulation.  Each lane owns one record at a time: it
 // forms the S u-weights K[jKer] and, per footprint row, the v-weight K[kKer]
@@ -482,7 +523,7 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, const uint32_t *__restrict__
     if (blockIdx.x >= meta[1]) return;
     const uint4 task = tasks[blockIdx.x];
     const uint32_t b = task.x, r0 = task.y, r1 = task.z;
-    const bool split = task.w != 0;
+    const bool split = task.w == 1u;
 
     const uint32_t per_plane = (uint32_t)g.tiles_u * (uint32_t)g.tiles_v;
     const int w = (int)(b / per_plane), rem = (int)(b % per_plane);
@@ -490,6 +531,26 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, const uint32_t *__restrict__
     const int j0 = tu * TU, k0 = tv * TV;
     const int nj = min(TU, g.nx - j0), nk = min(TV, g.ny - k0);
     T *grid = static_cast<T *>(grid_v);
+
+    if (task.w == kZeroTask) {
+        // A run of task.y empty buckets from b, overwrite mode: clear their tiles.
+        T2 z;
+        z.x = T(0);
+        z.y = T(0);
+        T2 *g2 = reinterpret_cast<T2 *>(grid);
+        for (uint32_t e = 0; e < r0; ++e) {
+            const uint32_t be = b + e;
+            const int we = (int)(be / per_plane), re = (int)(be % per_plane);
+            const int je = (re % g.tiles_u) * TU, ke = (re / g.tiles_u) * TV;
+            const int nje = min(TU, g.nx - je), nke = min(TV, g.ny - ke);
+            T2 *t0 = g2 + ((size_t)we * g.ny + (size_t)ke) * (size_t)g.nx + (size_t)je;
+            for (int idx = threadIdx.x; idx < nke * TU; idx += kTileThreads) {
+                const int kk = idx / TU, jj = idx % TU;
+                if (jj < nje) t0[(size_t)kk * g.nx + jj] = z;
+            }
+        }
+        return;
+    }
 
     for (int idx = threadIdx.x; idx < 2 * TV * LDU; idx += kTileThreads) lds[idx] = 0.0;
     for (int idx = threadIdx.x; idx < g.ntable; idx += kTileThreads) stab[idx] = (T)table[idx];
@@ -657,6 +718,7 @@ size_t grid_work_bytes(const GridGeom &g, uint32_t max_points, int precision)
     b += align_up(sizeof(uint32_t) * split_cap, 256);      // split_list
     b += align_up(sizeof(uint32_t) * 4, 256);              // meta
     b += align_up(rs * rec_cap, 256);                      // records
+    b += align_up(sizeof(uint32_t) * (size_t)g.planes * g.tiles_v, 256);  // band
     return b;
 }
 
@@ -680,6 +742,7 @@ void grid_work_carve(const GridGeom &g, uint32_t max_points, int precision, void
     w->split_list = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * split_cap));
     w->meta = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * 4));
     w->records = take(rs * rec_cap);
+    w->band = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * (size_t)g.planes * g.tiles_v));
     w->rec_cap = rec_cap;
     w->split_cap = split_cap;
     w->task_cap = task_cap;
@@ -690,50 +753,29 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
                        const double *table_d, uint32_t n, uint32_t F, uint32_t P,
                        const double *uu, const double *vv, const double *ww, const float *vr,
                        const float *vi, const float *wt, void *grid, int accumulate,
-                       hipStream_t s, Timing *tm, const SideStream *side, bool clear_empty)
+                       hipStream_t s, Timing *tm, bool clear_empty)
 {
     Timing off;
     if (tm == nullptr) tm = &off;
     const uint32_t nblk = (n + kRowsPerBlock - 1) / kRowsPerBlock;
+    const uint32_t cblk = (n + kBinRows - 1) / kBinRows;
     const uint32_t fill_blocks = min((g.nb + 255u) / 256u, 4096u);
+    // Overwrite of a dense grid: runs of empty buckets get zero tasks.
+    const int zero = (!accumulate && clear_empty) ? 1 : 0;
     tm->begin(RICK_STAGE_COUNT_ROWS, s);
     hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks), dim3(256), 0, s, w.counts, g.nb, 0u);
     if (n > 0)
-        hipLaunchKernelGGL(k_count, dim3(nblk), dim3(kThreads), 0, s, g, n, uu, vv, ww, w.counts);
+        hipLaunchKernelGGL(k_count, dim3(cblk), dim3(kBinThreads), 0, s, g, n, uu, vv, ww,
+                           w.counts);
     tm->end(RICK_STAGE_COUNT_ROWS, s);
-    // Empty tiles depend only on the histogram: clear them on the side stream
-    // while the scan, scatter and tile kernels run (disjoint cells).
-    hipStream_t zs = s;
-    const bool zero_empty = !accumulate && clear_empty;
-    if (zero_empty && side && side->s) {
-        hipError_t e = hipEventRecord(side->fork, s);
-        if (e == hipSuccess) e = hipStreamWaitEvent(side->s, side->fork, 0);
-        if (e != hipSuccess) return e;
-        zs = side->s;
-    }
-    if (zero_empty) {
-        const uint32_t zb = min(g.nb, 8192u);
-        tm->begin(RICK_STAGE_ZERO, zs);
-        if (precision == RICK_FP64)
-            hipLaunchKernelGGL(k_zero_empty<double2>, dim3(zb), dim3(kThreads), 0, zs, g, w.counts,
-                               static_cast<double2 *>(grid));
-        else
-            hipLaunchKernelGGL(k_zero_empty<float2>, dim3(zb), dim3(kThreads), 0, zs, g, w.counts,
-                               static_cast<float2 *>(grid));
-        tm->end(RICK_STAGE_ZERO, zs);
-        if (zs != s) {
-            hipError_t e = hipEventRecord(side->join, zs);
-            if (e != hipSuccess) return e;
-        }
-    }
     tm->begin(RICK_STAGE_SCAN, s);
     hipLaunchKernelGGL(k_scan_reduce, dim3(w.nscan), dim3(kThreads), 0, s, g, w.counts,
-                       w.blocksum, w.nscan);
+                       w.blocksum, w.nscan, zero);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kTopThreads), 0, s, g, w.blocksum, w.nscan,
                        w.offsets, w.meta);
-    hipLaunchKernelGGL(k_scan_down, dim3(w.nscan), dim3(kThreads), 0, s, g, w.counts, w.blocksum,
-                       w.nscan, w.offsets, w.cursor, w.tasks, w.task_cap, w.split_list,
-                       w.split_cap);
+    hipLaunchKernelGGL(k_scan_down, dim3(w.nscan), dim3(kThreads), 0, s, g, w.counts,
+                       w.blocksum, w.nscan, w.offsets, w.cursor, w.tasks, w.task_cap,
+                       w.split_list, w.split_cap, zero);
     tm->end(RICK_STAGE_SCAN, s);
     tm->begin(RICK_STAGE_SCATTER, s);
     if (n > 0) {
@@ -757,12 +799,9 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
                                w.split_list, w.meta, static_cast<float *>(grid));
         tm->end(RICK_STAGE_ZERO_SPLIT, s);
     }
-    hipError_t e = precision == RICK_FP64
-                       ? launch_tile<RICK_FP64>(g, w, table_d, grid, accumulate, s, tm)
-                       : launch_tile<RICK_FP32>(g, w, table_d, grid, accumulate, s, tm);
-    if (e != hipSuccess) return e;
-    if (zs != s) return hipStreamWaitEvent(s, side->join, 0);
-    return hipSuccess;
+    return precision == RICK_FP64
+               ? launch_tile<RICK_FP64>(g, w, table_d, grid, accumulate, s, tm)
+               : launch_tile<RICK_FP32>(g, w, table_d, grid, accumulate, s, tm);
 }
 
 }  // namespace rick

@@ -67,6 +67,7 @@ struct GridWork {
     uint32_t *split_list;   // [split_cap] buckets handled by more than one task
     uint32_t *meta;         // [4]: total records, tasks, split buckets, non-empty buckets
     void *records;          // [rec_cap] sorted row records
+    uint32_t *band;         // [planes*tiles_v] tile rows holding an occupied tile
     uint32_t rec_cap, split_cap, task_cap, nscan;
 };
 
@@ -75,18 +76,11 @@ void grid_work_carve(const GridGeom &g, uint32_t max_points, int precision, void
                      GridWork *w);
 
 // Launches the binned gridder.  Returns a hipError_t.
-// Side stream of a plan: work that only depends on the bucket histogram
-// (clearing empty tiles) is forked onto it and joined back before return.
-struct SideStream {
-    hipStream_t s = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
-};
 hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
                        const double *table_d, uint32_t num_points, uint32_t F, uint32_t P,
                        const double *uu, const double *vv, const double *ww,
                        const float *vr, const float *vi, const float *wt, void *grid,
-                       int accumulate, hipStream_t s, Timing *tm, const SideStream *side,
-                       bool clear_empty = true);
+                       int accumulate, hipStream_t s, Timing *tm, bool clear_empty = true);
 
 // FFT: per-plane 2-D backward transform of planes x ny x nx complex, scaled.
 struct FftPlan {
@@ -111,12 +105,13 @@ hipError_t launch_phase(const PhaseGeom &p, int precision, const void *gridss, d
 
 // Image chain after gridding, in place on grid (clobbered) and without
 // scratch: row FFT reading only the tiles whose gridder bucket count
-// (occ_counts, geometry occ_geom; null: dense grid) is non-zero, column FFT
-// whose last pass is fused with the phase correction and plane sum into the
-// images (overwritten).  Scales by `scale` (1/(Nx*Ny)).
+// (occ_counts, geometry occ_geom; null: dense grid) is non-zero and skipping
+// rows of tile rows without any (flags written to occ_band, planes*tiles_v),
+// column FFT whose last pass is fused with the phase correction and plane sum
+// into the images (overwritten).  Scales by `scale` (1/(Nx*Ny)).
 hipError_t launch_image_chain(const FftPlan &p, int precision, void *grid, double scale,
                               hipStream_t s, Timing *tm, const GridGeom *occ_geom,
-                              const uint32_t *occ_counts, const PhaseGeom &pg, double *re,
-                              double *im);
+                              const uint32_t *occ_counts, uint32_t *occ_band, const PhaseGeom &pg,
+                              double *re, double *im);
 
 }  // namespace rick

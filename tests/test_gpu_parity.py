@@ -72,6 +72,36 @@ def test_grid_tiles_and_split_buckets(tile, rows_per_task):
     assert rel_l2(got, z["grid"]) <= FP64_TOL
 
 
+@pytest.mark.parametrize("precision", [FP64, FP32])
+def test_grid_scattered_rows_overflow_block_tables(precision):
+    """20,000 rows spread uniformly over 16 planes of 1024^2 in 32x32 tiles:
+    each 8,192-row block touches thousands of distinct buckets, more than its
+    hash table holds, so many records take the per-bucket overflow range;
+    also spans three row blocks and a partial last block.  Includes rows at
+    w = 1.0 (dropped) and NaN visibilities (skipped)."""
+    rng = np.random.default_rng(123)
+    n, F, P, nx, planes = 20000, 2, 1, 1024, 16
+    uu = rng.uniform(0.0, 1.0, n)
+    vv = rng.uniform(0.0, 1.0, n)
+    ww = rng.uniform(0.0, 1.0, n)
+    ww[::97] = 1.0
+    vr = rng.normal(size=n * F * P).astype(np.float32)
+    vi = rng.normal(size=n * F * P).astype(np.float32)
+    vr[::53] = np.nan
+    wt = rng.uniform(0.5, 1.5, n * P).astype(np.float32)
+    ref = O.wstack(O.KAISERBESSEL, planes, uu, vv, ww, vr, vi, wt, F, P, 1.0 / nx, 1.0 / planes,
+                   7, nx, nx, threads=8)
+    print(f"case scattered precision={precision}", flush=True)
+    plan = Plan(nx, nx, planes, kernel=rick_amd.KAISERBESSEL, support=7, precision=precision,
+                max_points=n, tile=(32, 32))
+    ins = [dev(a) for a in (uu, vv, ww, vr, vi, wt)]
+    got = host(plan.grid(*ins, F, P, out=nan_grid(plan)))
+    assert np.isfinite(got).all()
+    assert rel_l2(got, ref) <= (FP64_TOL if precision == FP64 else 1e-6)
+    st = plan.grid_stats()
+    assert st["records"] >= n - n // 97 - 1
+
+
 def test_grid_accumulate_and_empty_input():
     z = load_golden("wstack_gauss_s7.npz")
     plan = grid_plan(z)
