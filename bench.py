@@ -58,6 +58,21 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic(kernel, workload, precision, mode="dense"):
+    """HBM bytes per dispatch of `kernel` from the committed PMC summary
+    profiles/<kernel>_<workload>_traffic.txt (FETCH_SIZE/WRITE_SIZE runs scaled
+    by the copy-probe factors), or None when there is none."""
+    path = ROOT / "profiles" / f"k_{kernel}_{workload}_traffic.txt"
+    try:
+        for line in path.read_text().splitlines():
+            f = line.split()
+            if len(f) == 4 and not line.startswith("#") and f[0] == precision and f[1] == mode:
+                return int(f[2]) + int(f[3])
+    except OSError:
+        pass
+    return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -270,7 +285,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-                     "traffic": None,
+                     "traffic": measured_traffic(dom, cfg.name, main_path),
+                     "traffic_source": f"profiles/k_{dom}_{cfg.name}_traffic.txt",
                      "alg_bytes_per_launch": alg,
                      "ms_per_launch": grid_stages[dom]["ms_per_launch"]},
         "gridding_stage": {"alg_bytes": stage_alg,
