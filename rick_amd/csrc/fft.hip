@@ -223,7 +223,34 @@ struct GlobalSink {
     }
 };
 
-template <typename T, int L, int G, bool ROWS, int P, typename Sink>
+// First-pass input of this thread: element n of line g for butterfly slot
+// i*R + r, or zero where the line does not exist or the input is known zero.
+template <typename T, int L, int G, bool ROWS>
+__device__ __forceinline__ void load_first(C2<T> *vals, int tid, const Lines &ln,
+                                           const C2<T> *gin)
+{
+    constexpr int R = radix_of(L, 0), LR = L / R, BPT = kE / R;
+    constexpr int NT = G * L / kE;
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+        int g, j;
+        bmap<ROWS, LR, G>(tid + i * NT, g, j);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int n = j + r * LR;
+            const bool live =
+                g < ln.nvalid && (ln.occ == nullptr || ln.occ[g * ln.occ_stride + n / ln.occ_tu]) &&
+                (ln.rowlive == nullptr ||
+                 ln.rowlive[(ln.rl_row0 + n * ln.rl_rstride) >> ln.rl_shift]);
+            vals[i * R + r] = live ? gin[ln.in_base + (size_t)g * ln.in_line + (size_t)n * ln.in_el]
+                                   : C2<T>{T(0), T(0)};
+        }
+    }
+}
+
+// One pass of the transform and, recursively, the rest.  With PRE the first
+// pass's inputs are already in vals (load_first), so it issues no loads.
+template <typename T, int L, int G, bool ROWS, int P, typename Sink, bool PRE = false>
 __device__ __forceinline__ void pass(C2<T> *vals, int tid, const Lines &ln, int ts,
                                      const C2<T> *__restrict__ tw, C2<T> *lds, const C2<T> *gin,
                                      Sink &sink)
@@ -234,6 +261,9 @@ __device__ __forceinline__ void pass(C2<T> *vals, int tid, const Lines &ln, int 
         constexpr int NT = G * L / kE;
         constexpr int LP = lpad(L) + 1;
         constexpr bool from_lds = P > 0, to_lds = P + 1 < NP;
+        if constexpr (!from_lds && PRE) {
+            // inputs preloaded
+        } else {
 #pragma unroll
         for (int i = 0; i < BPT; ++i) {
             int g, j;
@@ -254,6 +284,7 @@ __device__ __forceinline__ void pass(C2<T> *vals, int tid, const Lines &ln, int 
                                 : C2<T>{T(0), T(0)};
                 }
             }
+        }
         }
         if constexpr (from_lds) __syncthreads();
 #pragma unroll
@@ -428,6 +459,7 @@ k_fft_cols(ColArgs a, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gou
 template <typename T>
 struct PhaseSink {
     double are[kE], aim[kE];
+    double nfv[kE];  // FP64: n of each of this thread's pixels (set once)
     const double *ltab, *mtab;
     double wterm, dwnorm;
     int u0, s, vs_mul, v_stride, nvalid;
@@ -440,10 +472,12 @@ struct PhaseSink {
             aim[idx] += (double)o.y;
             return;
         }
-        const int u = u0 + g, v = s * vs_mul + n * v_stride;
-        const double nf = pixel_n(ltab[u], mtab[v]);
-        if constexpr (sizeof(T) == 8) phase_add_f64(o.x, o.y, wterm, nf, dwnorm, are[idx], aim[idx]);
-        else phase_add_f32(o.x, o.y, wterm, nf, dwnorm, are[idx], aim[idx]);
+        if constexpr (sizeof(T) == 8) {
+            phase_add_f64(o.x, o.y, wterm, nfv[idx], dwnorm, are[idx], aim[idx]);
+        } else {
+            const int u = u0 + g, v = s * vs_mul + n * v_stride;
+            phase_add_f32(o.x, o.y, wterm, pixel_n(ltab[u], mtab[v]), dwnorm, are[idx], aim[idx]);
+        }
     }
 };
 
@@ -479,21 +513,61 @@ k_fft_cols_phase(ColArgs a, int planes, const C2<T> *__restrict__ tw,
     ln.tw_set = 0;
     ln.occ = nullptr;
     ln.rowlive = nullptr;
-    for (int iw = 0; iw < planes; ++iw) {
-        const size_t plane = (size_t)iw * (size_t)a.nx * (size_t)a.ny;
-        if (co.band) {
-            col_rowlive(co, iw, s * a.in_set_mul, a.in_stride, rl, ln);
-        }
-        ln.in_base = plane + (size_t)s * a.in_set_mul * (size_t)a.nx + c0;
-        ln.out_base = 0;
-        sink.wterm = pg.wmin + iw * pg.dw;  // phase_correction.cu:200
-        C2<T> vals[kE];
-        pass<T, L, G, false, 0>(vals, threadIdx.x, ln, a.ts, tw, lds, grid, sink);
-        __syncthreads();  // LDS is reused by the next plane
-    }
-    // Write the accumulated pixels: the last pass's element layout.
+    ln.out_base = 0;
+    // Pixels of this thread in the last pass's element layout.
     constexpr int P = npasses(L) - 1, R = radix_of(L, P), Ns = ns_of(L, P), LR = L / R,
                   BPT = kE / R;
+    if constexpr (sizeof(T) == 8) {
+        // FP64: n of every pixel once, not once per plane (FP32 keeps the
+        // registers for occupancy and recomputes it).
+#pragma unroll
+        for (int i = 0; i < BPT; ++i) {
+            int g, j;
+            bmap<false, LR, G>(threadIdx.x + i * NT, g, j);
+            const int dbase = (j / Ns) * Ns * R + j % Ns;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int n = dbase + r * Ns;
+                sink.nfv[i * R + r] =
+                    g < sink.nvalid ? pixel_n(pg.ltab[c0 + g], pg.mtab[s * vs_mul + n * v_stride])
+                                    : 1.0;
+            }
+        }
+    }
+    auto plane_lines = [&](int iw) {
+        if (co.band) col_rowlive(co, iw, s * a.in_set_mul, a.in_stride, rl, ln);
+        ln.in_base = (size_t)iw * (size_t)a.nx * (size_t)a.ny +
+                     (size_t)s * a.in_set_mul * (size_t)a.nx + c0;
+    };
+    // FP64 runs one wave per SIMD at this register count: the next plane's
+    // inputs are loaded into registers while this plane is transformed.  (The
+    // FP32 variant keeps two waves per SIMD instead, which measured faster.)
+    constexpr bool kPrefetch = sizeof(T) == 8;
+    C2<T> vals[kE];
+    if constexpr (kPrefetch) {
+        plane_lines(0);
+        load_first<T, L, G, false>(vals, threadIdx.x, ln, grid);
+    }
+    for (int iw = 0; iw < planes; ++iw) {
+        sink.wterm = pg.wmin + iw * pg.dw;  // phase_correction.cu:200
+        if constexpr (kPrefetch) {
+            C2<T> nxt[kE];
+            if (iw + 1 < planes) {
+                plane_lines(iw + 1);
+                load_first<T, L, G, false>(nxt, threadIdx.x, ln, grid);
+            }
+            pass<T, L, G, false, 0, PhaseSink<T>, true>(vals, threadIdx.x, ln, a.ts, tw, lds,
+                                                         grid, sink);
+            __syncthreads();  // LDS is reused by the next plane
+#pragma unroll
+            for (int e = 0; e < kE; ++e) vals[e] = nxt[e];
+        } else {
+            plane_lines(iw);
+            pass<T, L, G, false, 0>(vals, threadIdx.x, ln, a.ts, tw, lds, grid, sink);
+            __syncthreads();  // LDS is reused by the next plane
+        }
+    }
+    // Write the accumulated pixels.
 #pragma unroll
     for (int i = 0; i < BPT; ++i) {
         int g, j;
