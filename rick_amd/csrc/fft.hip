@@ -459,7 +459,9 @@ k_fft_cols(ColArgs a, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gou
 template <typename T>
 struct PhaseSink {
     double are[kE], aim[kE];
-    double nfv[kE];  // FP64: n of each of this thread's pixels (set once)
+    double nfv[kE];       // FP64: n of each of this thread's pixels (set once)
+    const double *nfl;    // FP32: n of the workgroup's pixels in LDS, [n * G + g]
+    int G;
     const double *ltab, *mtab;
     double wterm, dwnorm;
     int u0, s, vs_mul, v_stride, nvalid;
@@ -475,8 +477,7 @@ struct PhaseSink {
         if constexpr (sizeof(T) == 8) {
             phase_add_f64(o.x, o.y, wterm, nfv[idx], dwnorm, are[idx], aim[idx]);
         } else {
-            const int u = u0 + g, v = s * vs_mul + n * v_stride;
-            phase_add_f32(o.x, o.y, wterm, pixel_n(ltab[u], mtab[v]), dwnorm, are[idx], aim[idx]);
+            phase_add_f32(o.x, o.y, wterm, nfl[n * G + g], dwnorm, are[idx], aim[idx]);
         }
     }
 };
@@ -491,6 +492,7 @@ k_fft_cols_phase(ColArgs a, int planes, const C2<T> *__restrict__ tw,
     constexpr int NT = G * L / kE;
     __shared__ C2<T> lds[lds_elems(L, false)];
     __shared__ unsigned char rl[256];
+    __shared__ double nfl[sizeof(T) == 4 ? G * L : 1];
     const int c0 = blockIdx.x * G;
     const int s = blockIdx.y;
     PhaseSink<T> sink;
@@ -517,9 +519,19 @@ k_fft_cols_phase(ColArgs a, int planes, const C2<T> *__restrict__ tw,
     // Pixels of this thread in the last pass's element layout.
     constexpr int P = npasses(L) - 1, R = radix_of(L, P), Ns = ns_of(L, P), LR = L / R,
                   BPT = kE / R;
-    if constexpr (sizeof(T) == 8) {
-        // FP64: n of every pixel once, not once per plane (FP32 keeps the
-        // registers for occupancy and recomputes it).
+    if constexpr (sizeof(T) == 4) {
+        // FP32: n of every pixel once, in LDS (registers are kept for two
+        // waves per SIMD).
+        for (int e = threadIdx.x; e < G * L; e += NT) {
+            const int g = e % G, n = e / G;
+            nfl[e] = g < sink.nvalid ? pixel_n(pg.ltab[c0 + g], pg.mtab[s * vs_mul + n * v_stride])
+                                     : 1.0;
+        }
+        sink.nfl = nfl;
+        sink.G = G;
+        __syncthreads();
+    } else {
+        // FP64: n of every pixel once, in registers.
 #pragma unroll
         for (int i = 0; i < BPT; ++i) {
             int g, j;

@@ -78,17 +78,17 @@ __device__ __forceinline__ void phase_add_f64(double p, double q, double wterm, 
     aim += (p * sn + q * cs) * dwnorm * nf;
 }
 
-// FP32 plane data: the phase argument and its reduction to [-pi, pi] stay
-// FP64, the rotation is FP32, the accumulation FP64.
+// FP32 plane data: the phase argument and its reduction to one turn stay
+// FP64; the rotation is FP32 (the hardware sine/cosine of a fraction of a
+// turn in [-1/2, 1/2], absolute error a few 1e-7), the accumulation FP64.
 __device__ __forceinline__ void phase_add_f32(float p, float q, double wterm, double nf,
                                               double dwnorm, double &are, double &aim)
 {
 #pragma clang fp contract(off)
     const double arg = 2.0 * kRefPi * wterm * (nf - 1.0);
-    const double k = rint(arg * (1.0 / (2.0 * 3.141592653589793)));
-    const float red = (float)(arg - k * (2.0 * 3.141592653589793));
-    float sn, cs;
-    sincosf(red, &sn, &cs);
+    const double turns = arg * (1.0 / (2.0 * 3.141592653589793));
+    const float f = (float)(turns - rint(turns));
+    const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
     const float fre = (p * cs - q * sn), fim = (p * sn + q * cs);
     are += (double)fre * dwnorm * nf;
     aim += (double)fim * dwnorm * nf;
