@@ -207,6 +207,11 @@ int rick_plan_create(rick_plan *out, const rick_plan_desc *desc)
     // Host tables -> device (synchronous; plan creation is not on a hot path).
     std::vector<double> h(kMaxTable, 0.0);
     if (d.kernel != RICK_GAUSS_HI_PRECISION) kernel_table(d.kernel, d.w_support, h.data());
+    // Largest per-cell weight: K[j]*K[k] <= max|K|^2, or the hi-precision
+    // Gaussian's norm (w-stacking.cu:254-256, 496).
+    double kmax = 0.0;
+    for (int i = 0; i < g.ntable; ++i) kmax = std::max(kmax, std::fabs(h[i]));
+    g.cmax = d.kernel == RICK_GAUSS_HI_PRECISION ? (1.0 / (2.0 * 1.0 * 1.0)) / kRefPi : kmax * kmax;
     e = hipMemcpy(p->table_d, h.data(), sizeof(double) * kMaxTable, hipMemcpyHostToDevice);
     std::vector<double> tw;
     auto put_tw = [&](int n, size_t o) {

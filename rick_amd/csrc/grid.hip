@@ -499,14 +499,30 @@ k_zero_split(GridGeom g, const uint32_t *__restrict__ split_list, const uint32_t
 }
 
 
+// 2^52 + 2^51: adding it to x with |x| < 2^51 leaves round(x) in the low
+// mantissa bits, so bits(x + M) - bits(M) is round(x) as a two's-complement
+// integer (one FMA and one integer subtraction per value).
+__device__ __forceinline__ unsigned long long to_fixed(double x, double scale)
+{
+    constexpr double M = 6755399441055744.0;
+    return (unsigned long long)(__double_as_longlong(__fma_rn(x, scale, M)) -
+                                __double_as_longlong(M));
+}
+
 // Per-task LDS tile accumulation.  Each lane owns one record at a time: it
 // forms the S u-weights K[jKer] and, per footprint row, the v-weight K[kKer]
 // (the reference's index arithmetic, evaluated 2S times per record instead
 // of twice per cell), then adds S*(K[jKer]*K[kKer]) to the S x S cells with
 // non-returning LDS atomics, so the 64 lanes of a wave stream independent
-// updates.  Both paths accumulate in an FP64 LDS tile; in the FP32 path the
-// per-cell products are FP32 and the tile is rounded to FP32 once, when it is
-// written to the grid.  SMAX >= support bounds the unrolled footprint loops.
+// updates.  The FP64 path accumulates in an FP64 LDS tile (ds_add_f64).  The
+// FP32 path accumulates 64-bit fixed point (ds_add_u64, about twice the
+// throughput of ds_add_f64 here, and order-independent): the task first sums
+// |S| over its records, which with the largest cell weight bounds every
+// cell's sum by B; values are scaled by a power of two that maps B to 2^50,
+// so each contribution is rounded to B * 2^-50 (far below FP32 precision of
+// any cell that matters to the image) and the tile is rounded to FP32 once
+// when written.  A non-finite bound (NaN/Inf data) falls back to FP64
+// atomics.  SMAX >= support bounds the unrolled footprint loops.
 template <int PREC, int TU, int TV, int SMAX>
 __global__ void __launch_bounds__(kTileThreads)
 k_tile(GridGeom g, const uint4 *__restrict__ tasks, const uint32_t *__restrict__ meta,
@@ -554,6 +570,39 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, const uint32_t *__restrict__
 
     for (int idx = threadIdx.x; idx < 2 * TV * LDU; idx += kTileThreads) lds[idx] = 0.0;
     for (int idx = threadIdx.x; idx < g.ntable; idx += kTileThreads) stab[idx] = (T)table[idx];
+    // FP32: fixed-point scale from the bound of the task's cell sums.
+    bool fixed = false;
+    double scale = 1.0;
+    if constexpr (PREC == RICK_FP32) {
+        __shared__ double red[2][kTileThreads / 64];
+        double ar = 0.0, ai = 0.0;
+        for (uint32_t r = r0 + threadIdx.x; r < r1; r += kTileThreads) {
+            ar += fabs((double)rec[r].sre);
+            ai += fabs((double)rec[r].sim);
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            ar += __shfl_xor(ar, o);
+            ai += __shfl_xor(ai, o);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            red[0][threadIdx.x >> 6] = ar;
+            red[1][threadIdx.x >> 6] = ai;
+        }
+        __syncthreads();
+        double br = 0.0, bi = 0.0;
+#pragma unroll
+        for (int q = 0; q < kTileThreads / 64; ++q) {
+            br += red[0][q];
+            bi += red[1][q];
+        }
+        const double bound = fmax(br, bi) * g.cmax * 1.01;  // margin for FP32 products
+        fixed = isfinite(br) && isfinite(bi) && isfinite(bound);  // (fmax drops NaNs)
+        if (fixed && bound > 0.0) {
+            int e;
+            frexp(bound, &e);                    // bound < 2^e
+            scale = ldexp(1.0, 50 - e);          // bound * scale < 2^50
+        }
+    }
     __syncthreads();
 
     const int S = g.support, KL = g.kl;
@@ -619,10 +668,21 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, const uint32_t *__restrict__
                                                      std22))
                                    : wu[dj] * wv;
                 const T are = s_re * c, aim = s_im * c;
-                __hip_atomic_fetch_add(&sre[rowbase + dj], (double)are, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-                __hip_atomic_fetch_add(&sim[rowbase + dj], (double)aim, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (PREC == RICK_FP32 && fixed) {
+                    __hip_atomic_fetch_add(
+                        reinterpret_cast<unsigned long long *>(&sre[rowbase + dj]),
+                        to_fixed((double)are, scale), __ATOMIC_RELAXED,
+                        __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_add(
+                        reinterpret_cast<unsigned long long *>(&sim[rowbase + dj]),
+                        to_fixed((double)aim, scale), __ATOMIC_RELAXED,
+                        __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else {
+                    __hip_atomic_fetch_add(&sre[rowbase + dj], (double)are, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_add(&sim[rowbase + dj], (double)aim, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
             }
         }
     }
@@ -632,7 +692,15 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, const uint32_t *__restrict__
         const int kk = idx / TU, jj = idx % TU;
         if (jj >= nj) continue;
         const int li = kk * LDU + jj;
-        const T vre = (T)sre[li], vim = (T)sim[li];
+        T vre, vim;
+        if (PREC == RICK_FP32 && fixed) {
+            const double inv = 1.0 / scale;  // exact: a power of two
+            vre = (T)((double)reinterpret_cast<const long long *>(sre)[li] * inv);
+            vim = (T)((double)reinterpret_cast<const long long *>(sim)[li] * inv);
+        } else {
+            vre = (T)sre[li];
+            vim = (T)sim[li];
+        }
         const size_t c = ((size_t)w * g.ny + (size_t)(k0 + kk)) * (size_t)g.nx + (size_t)(j0 + jj);
         if (split) {
             if (vre != T(0)) unsafeAtomicAdd(&grid[2 * c], vre);

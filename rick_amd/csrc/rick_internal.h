@@ -55,6 +55,7 @@ struct GridGeom {
     uint32_t nb;            // buckets = planes*tiles_u*tiles_v
     uint32_t rmax;          // rows per task
     int ntable;
+    double cmax;            // largest per-cell convolution weight (a bound)
 };
 
 // Workspace of the binned gridder, carved out of one device allocation.

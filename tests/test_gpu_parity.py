@@ -102,6 +102,31 @@ def test_grid_scattered_rows_overflow_block_tables(precision):
     assert st["records"] >= n - n // 97 - 1
 
 
+@pytest.mark.parametrize("precision", [FP64, FP32])
+def test_grid_nan_imaginary_part_propagates(precision):
+    """Only the real part is tested for NaN (w-stacking.cu:517): a NaN
+    imaginary part with a finite real part poisons the imaginary grid cells of
+    that row's footprint, as in the reference.  (In the FP32 path this takes
+    the FP64-atomic fallback of the fixed-point tile accumulation.)"""
+    rng = np.random.default_rng(7)
+    n, nx, planes = 3000, 256, 4
+    uu, vv, ww = rng.uniform(0.1, 0.9, n), rng.uniform(0.1, 0.9, n), rng.uniform(0.0, 0.99, n)
+    vr = rng.normal(size=n).astype(np.float32)
+    vi = rng.normal(size=n).astype(np.float32)
+    vi[17] = np.nan
+    wt = np.ones(n, dtype=np.float32)
+    ref = O.wstack(O.GAUSS, planes, uu, vv, ww, vr, vi, wt, 1, 1, 1.0 / nx, 1.0 / planes, 7, nx,
+                   nx, threads=4)
+    print(f"case nan-imag precision={precision}", flush=True)
+    plan = Plan(nx, nx, planes, kernel=rick_amd.GAUSS, support=7, precision=precision,
+                max_points=n)
+    got = host(plan.grid(*[dev(a) for a in (uu, vv, ww, vr, vi, wt)], 1, 1, out=nan_grid(plan)))
+    assert np.array_equal(np.isnan(got), np.isnan(ref))
+    assert np.isnan(ref).sum() > 0
+    ok = ~np.isnan(ref)
+    assert rel_l2(got[ok], ref[ok]) <= (FP64_TOL if precision == FP64 else 1e-6)
+
+
 def test_grid_accumulate_and_empty_input():
     z = load_golden("wstack_gauss_s7.npz")
     plan = grid_plan(z)
