@@ -19,7 +19,7 @@
 //   k_scatter    forms per row S = sum_{f,p} fl32(weight*vis) (NaN real parts
 //                skipped, w-stacking.cu:517) and writes a record {pos_u,
 //                pos_v, S} to each of its buckets; slots come from one
-//                returning global atomic per distinct key per 1024-row block.
+//                returning global atomic per distinct key per 4096-row block.
 //   k_zero_split clears the tiles of split buckets (overwrite mode).
 //   k_tile       one workgroup per task: accumulates its records into an FP64
 //                LDS tile (one lane per record, LDS atomics), then writes
@@ -46,11 +46,14 @@ namespace {
 constexpr int kThreads = 256;                 // count / scatter / scan workgroups
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int kScanBlock = 4 * kThreads;      // buckets per scan block
-// Scatter: 1024-row blocks whose distinct keys always fit the LDS hash table.
+// Scatter: 4096-row blocks of 1024 threads, 4 rows in flight per thread;
+// keys beyond the hash table's reach take a per-record global atomic.
+constexpr int kScatterThreads = 1024;
 constexpr int kRowsPerThread = 4;
-constexpr int kRowsPerBlock = kThreads * kRowsPerThread;
+constexpr int kRowsPerBlock = kScatterThreads * kRowsPerThread;
 constexpr int kHashBits = 12;
-constexpr int kHashSlots = 1 << kHashBits;    // >= distinct keys of one block (4 x 1024)
+constexpr int kHashSlots = 1 << kHashBits;
+constexpr uint32_t kDirect = 0x80000000u;     // item flag: key without a hash slot
 // Count: 8192-row blocks, bounded probing, overflow -> direct global atomic.
 constexpr int kBinThreads = 512;
 constexpr int kBinRowsPerThread = 16;
@@ -145,18 +148,6 @@ __device__ __forceinline__ uint32_t hinsert_bounded(uint32_t *hkey, uint32_t key
     }
     return kEmpty;
 }
-
-// LDS open-addressing hash set of bucket keys.
-__device__ __forceinline__ uint32_t hinsert(uint32_t *hkey, uint32_t key)
-{
-    uint32_t h = (key * 2654435761u) >> (32 - kHashBits);
-    while (true) {
-        const uint32_t old = atomicCAS(&hkey[h], kEmpty, key);
-        if (old == kEmpty || old == key) return h;
-        h = (h + 1) & (kHashSlots - 1);
-    }
-}
-
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v)
 {
@@ -366,7 +357,7 @@ k_scan_down(GridGeom g, const uint32_t *__restrict__ counts,
 }
 
 template <int PREC>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kScatterThreads)
 k_scatter(GridGeom g, uint32_t n, uint32_t F, uint32_t P, const double *__restrict__ uu,
           const double *__restrict__ vv, const double *__restrict__ ww,
           const float *__restrict__ vr, const float *__restrict__ vi,
@@ -374,7 +365,7 @@ k_scatter(GridGeom g, uint32_t n, uint32_t F, uint32_t P, const double *__restri
           typename RecOf<PREC>::type *__restrict__ rec)
 {
     __shared__ uint32_t hkey[kHashSlots], hcnt[kHashSlots], hbase[kHashSlots];
-    for (int i = threadIdx.x; i < kHashSlots; i += kThreads) {
+    for (int i = threadIdx.x; i < kHashSlots; i += kScatterThreads) {
         hkey[i] = kEmpty;
         hcnt[i] = 0;
     }
@@ -387,7 +378,7 @@ k_scatter(GridGeom g, uint32_t n, uint32_t F, uint32_t P, const double *__restri
     int nk[kRowsPerThread];
 #pragma unroll
     for (int k = 0; k < kRowsPerThread; ++k) {
-        const uint32_t i = base + k * kThreads + threadIdx.x;
+        const uint32_t i = base + k * kScatterThreads + threadIdx.x;
         nk[k] = 0;
         pu[k] = pv[k] = sre[k] = sim[k] = 0.0;
         Foot f;
@@ -440,15 +431,19 @@ k_scatter(GridGeom g, uint32_t n, uint32_t F, uint32_t P, const double *__restri
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             if (q < nk[k]) {
-                const uint32_t slot = hinsert(hkey, keys[q]);
-                const uint32_t rank = atomicAdd(&hcnt[slot], 1u);
-                item[k][q] = slot | (rank << kHashBits);
+                const uint32_t slot = hinsert_bounded<kHashBits>(hkey, keys[q]);
+                if (slot == kEmpty) {
+                    item[k][q] = kDirect | keys[q];  // keys < 2^31
+                } else {
+                    const uint32_t rank = atomicAdd(&hcnt[slot], 1u);
+                    item[k][q] = slot | (rank << kHashBits);
+                }
             }
         }
     }
     __syncthreads();
     // One returning global atomic per distinct key, all issued together.
-    for (int i = threadIdx.x; i < kHashSlots; i += kThreads) {
+    for (int i = threadIdx.x; i < kHashSlots; i += kScatterThreads) {
         const uint32_t c = hcnt[i];
         if (c) hbase[i] = atomicAdd(&cursor[hkey[i]], c);
     }
@@ -459,7 +454,9 @@ k_scatter(GridGeom g, uint32_t n, uint32_t F, uint32_t P, const double *__restri
         for (int q = 0; q < 4; ++q) {
             if (q < nk[k]) {
                 const uint32_t it = item[k][q];
-                const uint32_t pos = hbase[it & (kHashSlots - 1)] + (it >> kHashBits);
+                const uint32_t pos = (it & kDirect)
+                                         ? atomicAdd(&cursor[it & ~kDirect], 1u)
+                                         : hbase[it & (kHashSlots - 1)] + (it >> kHashBits);
                 typename RecOf<PREC>::type r;
                 r.pu = pu[k];
                 r.pv = pv[k];
@@ -613,7 +610,13 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, const uint32_t *__restrict__
     // Cells of this tile a footprint may touch, intersected with the grid.
     const int jlo = j0, jhi = j0 + nj - 1, klo = k0, khi = k0 + nk - 1;
 
-    for (uint32_t r = r0 + threadIdx.x; r < r1; r += kTileThreads) {
+    // Record r0 + c + 8*lane + wave of each chunk c of 512: consecutive records
+    // (often one baseline at consecutive times, i.e. the same cells) go to
+    // different waves, so the lanes of one atomic instruction rarely collide.
+    const uint32_t roff = (threadIdx.x & 63u) * (kTileThreads / 64) + (threadIdx.x >> 6);
+    for (uint32_t c0 = r0; c0 < r1; c0 += kTileThreads) {
+        const uint32_t r = c0 + roff;
+        if (r >= r1) continue;
         const auto rc = rec[r];
         const double pu = rc.pu, pv = rc.pv;
         const T s_re = rc.sre, s_im = rc.sim;
@@ -848,11 +851,13 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
     tm->begin(RICK_STAGE_SCATTER, s);
     if (n > 0) {
         if (precision == RICK_FP64)
-            hipLaunchKernelGGL(k_scatter<RICK_FP64>, dim3(nblk), dim3(kThreads), 0, s, g, n, F, P,
+            hipLaunchKernelGGL(k_scatter<RICK_FP64>, dim3(nblk), dim3(kScatterThreads), 0, s, g, n,
+                               F, P,
                                uu, vv, ww, vr, vi, wt, w.cursor,
                                static_cast<RecF64 *>(w.records));
         else
-            hipLaunchKernelGGL(k_scatter<RICK_FP32>, dim3(nblk), dim3(kThreads), 0, s, g, n, F, P,
+            hipLaunchKernelGGL(k_scatter<RICK_FP32>, dim3(nblk), dim3(kScatterThreads), 0, s, g, n,
+                               F, P,
                                uu, vv, ww, vr, vi, wt, w.cursor,
                                static_cast<RecF32 *>(w.records));
     }
