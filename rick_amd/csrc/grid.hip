@@ -187,7 +187,12 @@ k_count(GridGeom g, uint32_t n, const double *__restrict__ uu, const double *__r
 }
 
 // Scan of the histogram in three kernels: per-block sums, a single-block scan
-// of those, and the per-block downsweep that emits offsets and tasks.
+// of those, and the per-block downsweep that emits offsets and tasks.  Five
+// quantities are scanned: records, accumulation tasks, split buckets,
+// non-empty buckets, zero tasks.  The task array holds the accumulation
+// tasks first and the zero tasks after them; the tile kernel interleaves the
+// two evenly over its workgroups.
+constexpr int kScanQ = 5;
 // Tasks of a bucket with c > 0 records.
 __device__ __forceinline__ uint32_t ntasks_of(uint32_t c, uint32_t rmax)
 {
@@ -197,28 +202,27 @@ __device__ __forceinline__ uint32_t ntasks_of(uint32_t c, uint32_t rmax)
 // Tasks of a scan thread's 4 consecutive buckets (nv of them exist): those of
 // the non-empty ones, plus, when the grid is overwritten (zero != 0), one zero
 // task per kZeroRun consecutive empty ones.
-__device__ __forceinline__ uint32_t group_tasks(const uint32_t *c, int nv, uint32_t rmax,
-                                                int zero)
+__device__ __forceinline__ void group_tasks(const uint32_t *c, int nv, uint32_t rmax, int zero,
+                                            uint32_t &real, uint32_t &zeros)
 {
-    uint32_t t = 0;
+    real = zeros = 0;
     int run = 0;  // length of the current run of empty buckets
     for (int k = 0; k < nv; ++k) {
         if (c[k]) {
-            t += ntasks_of(c[k], rmax);
+            real += ntasks_of(c[k], rmax);
             run = 0;
         } else {
-            if (zero && run % kZeroRun == 0) ++t;
+            if (zero && run % kZeroRun == 0) ++zeros;
             ++run;
         }
     }
-    return t;
 }
 
 __global__ void __launch_bounds__(kThreads)
 k_scan_reduce(GridGeom g, const uint32_t *__restrict__ counts, uint32_t *__restrict__ blocksum,
               uint32_t nscan, int zero)
 {
-    __shared__ uint32_t s[4][kThreads];
+    __shared__ uint32_t s[kScanQ][kThreads];
     uint32_t r = 0, sp = 0, ne = 0, c[4] = {0, 0, 0, 0};
     const uint32_t b0 = blockIdx.x * kScanBlock + threadIdx.x * 4;
     const int nv = b0 >= g.nb ? 0 : (int)min(4u, g.nb - b0);
@@ -231,22 +235,24 @@ k_scan_reduce(GridGeom g, const uint32_t *__restrict__ counts, uint32_t *__restr
             ne += c[k] > 0 ? 1u : 0u;
         }
     }
-    const uint32_t t = group_tasks(c, nv, g.rmax, zero);
+    uint32_t tr, tz;
+    group_tasks(c, nv, g.rmax, zero, tr, tz);
     s[0][threadIdx.x] = r;
-    s[1][threadIdx.x] = t;
+    s[1][threadIdx.x] = tr;
     s[2][threadIdx.x] = sp;
     s[3][threadIdx.x] = ne;
+    s[4][threadIdx.x] = tz;
     __syncthreads();
     for (int d = kThreads / 2; d > 0; d >>= 1) {
         if ((int)threadIdx.x < d) {
 #pragma unroll
-            for (int a = 0; a < 4; ++a) s[a][threadIdx.x] += s[a][threadIdx.x + d];
+            for (int a = 0; a < kScanQ; ++a) s[a][threadIdx.x] += s[a][threadIdx.x + d];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int a = 0; a < 4; ++a) blocksum[a * nscan + blockIdx.x] = s[a][0];
+        for (int a = 0; a < kScanQ; ++a) blocksum[a * nscan + blockIdx.x] = s[a][0];
     }
 }
 
@@ -256,29 +262,29 @@ __global__ void __launch_bounds__(kTopThreads)
 k_scan_top(GridGeom g, uint32_t *__restrict__ blocksum, uint32_t nscan,
            uint32_t *__restrict__ offsets, uint32_t *__restrict__ meta)
 {
-    __shared__ uint32_t s[4][kTopThreads];
-    uint32_t carry[4] = {0, 0, 0, 0};
+    __shared__ uint32_t s[kScanQ][kTopThreads];
+    uint32_t carry[kScanQ] = {0, 0, 0, 0, 0};
     for (uint32_t c0 = 0; c0 < nscan; c0 += kTopThreads) {
         const uint32_t i = c0 + threadIdx.x;
-        uint32_t v[4];
+        uint32_t v[kScanQ];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
+        for (int a = 0; a < kScanQ; ++a) {
             v[a] = i < nscan ? blocksum[a * nscan + i] : 0u;
             s[a][threadIdx.x] = v[a];
         }
         __syncthreads();
         for (int d = 1; d < kTopThreads; d <<= 1) {
-            uint32_t add[4] = {0, 0, 0, 0};
+            uint32_t add[kScanQ] = {0, 0, 0, 0, 0};
             if ((int)threadIdx.x >= d)
 #pragma unroll
-                for (int a = 0; a < 4; ++a) add[a] = s[a][threadIdx.x - d];
+                for (int a = 0; a < kScanQ; ++a) add[a] = s[a][threadIdx.x - d];
             __syncthreads();
 #pragma unroll
-            for (int a = 0; a < 4; ++a) s[a][threadIdx.x] += add[a];
+            for (int a = 0; a < kScanQ; ++a) s[a][threadIdx.x] += add[a];
             __syncthreads();
         }
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
+        for (int a = 0; a < kScanQ; ++a) {
             if (i < nscan) blocksum[a * nscan + i] = carry[a] + s[a][threadIdx.x] - v[a];
             carry[a] += s[a][kTopThreads - 1];
         }
@@ -286,10 +292,11 @@ k_scan_top(GridGeom g, uint32_t *__restrict__ blocksum, uint32_t nscan,
     }
     if (threadIdx.x == 0) {
         offsets[g.nb] = carry[0];
-        meta[0] = carry[0];
-        meta[1] = carry[1];
-        meta[2] = carry[2];
-        meta[3] = carry[3];
+        meta[0] = carry[0];             // records
+        meta[1] = carry[1] + carry[4];  // tasks (accumulation + zero)
+        meta[2] = carry[2];             // split buckets
+        meta[3] = carry[3];             // non-empty buckets
+        meta[4] = carry[1];             // accumulation tasks (stored first)
     }
 }
 
@@ -297,10 +304,11 @@ __global__ void __launch_bounds__(kThreads)
 k_scan_down(GridGeom g, const uint32_t *__restrict__ counts,
             const uint32_t *__restrict__ blockoff, uint32_t nscan, uint32_t *__restrict__ offsets,
             uint32_t *__restrict__ cursor, uint4 *__restrict__ tasks, uint32_t task_cap,
-            uint32_t *__restrict__ split_list, uint32_t split_cap, int zero)
+            uint32_t *__restrict__ split_list, uint32_t split_cap, const uint32_t *__restrict__ meta,
+            int zero)
 {
-    __shared__ uint32_t s[3][kThreads];
-    uint32_t c[4] = {0, 0, 0, 0}, v[3] = {0, 0, 0};
+    __shared__ uint32_t s[4][kThreads];
+    uint32_t c[4] = {0, 0, 0, 0}, v[4] = {0, 0, 0, 0};
     const uint32_t b0 = blockIdx.x * kScanBlock + threadIdx.x * 4;
     const int nv = b0 >= g.nb ? 0 : (int)min(4u, g.nb - b0);
 #pragma unroll
@@ -311,23 +319,25 @@ k_scan_down(GridGeom g, const uint32_t *__restrict__ counts,
             v[2] += c[k] > g.rmax ? 1u : 0u;
         }
     }
-    v[1] = group_tasks(c, nv, g.rmax, zero);
+    group_tasks(c, nv, g.rmax, zero, v[1], v[3]);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) s[a][threadIdx.x] = v[a];
+    for (int a = 0; a < 4; ++a) s[a][threadIdx.x] = v[a];
     __syncthreads();
     for (int d = 1; d < kThreads; d <<= 1) {
-        uint32_t add[3] = {0, 0, 0};
+        uint32_t add[4] = {0, 0, 0, 0};
         if ((int)threadIdx.x >= d)
 #pragma unroll
-            for (int a = 0; a < 3; ++a) add[a] = s[a][threadIdx.x - d];
+            for (int a = 0; a < 4; ++a) add[a] = s[a][threadIdx.x - d];
         __syncthreads();
 #pragma unroll
-        for (int a = 0; a < 3; ++a) s[a][threadIdx.x] += add[a];
+        for (int a = 0; a < 4; ++a) s[a][threadIdx.x] += add[a];
         __syncthreads();
     }
     uint32_t ro = blockoff[blockIdx.x] + s[0][threadIdx.x] - v[0];
     uint32_t to = blockoff[nscan + blockIdx.x] + s[1][threadIdx.x] - v[1];
     uint32_t so = blockoff[2 * nscan + blockIdx.x] + s[2][threadIdx.x] - v[2];
+    // Zero tasks follow all accumulation tasks (meta[4] of them).
+    uint32_t zo = meta[4] + blockoff[4 * nscan + blockIdx.x] + s[3][threadIdx.x] - v[3];
     int run = 0;
     for (int k = 0; k < nv; ++k) {
         const uint32_t b = b0 + k;
@@ -338,8 +348,8 @@ k_scan_down(GridGeom g, const uint32_t *__restrict__ counts,
             if (zero && run % kZeroRun == 0) {
                 uint32_t len = 1;
                 while (len < (uint32_t)kZeroRun && k + (int)len < nv && c[k + len] == 0) ++len;
-                if (to < task_cap) tasks[to] = make_uint4(b, len, 0u, kZeroTask);
-                ++to;
+                if (zo < task_cap) tasks[zo] = make_uint4(b, len, 0u, kZeroTask);
+                ++zo;
             }
             ++run;
         } else {
@@ -533,8 +543,14 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, const uint32_t *__restrict__
     __shared__ T stab[kMaxTable];
     double *sre = lds, *sim = lds + TV * LDU;
 
-    if (blockIdx.x >= meta[1]) return;
-    const uint4 task = tasks[blockIdx.x];
+    // Workgroup i runs accumulation task f(i) when f(i+1) > f(i), where
+    // f(i) = floor(i * Tr / T), else zero task Tr + i - f(i): the two kinds are
+    // spread evenly over the launch so that clearing overlaps accumulation.
+    const uint32_t ntask = meta[1], nreal = meta[4];
+    if (blockIdx.x >= ntask) return;
+    const uint32_t fi = (uint32_t)((uint64_t)blockIdx.x * nreal / ntask);
+    const uint32_t fn = (uint32_t)((uint64_t)(blockIdx.x + 1) * nreal / ntask);
+    const uint4 task = tasks[fn > fi ? fi : nreal + blockIdx.x - fi];
     const uint32_t b = task.x, r0 = task.y, r1 = task.z;
     const bool split = task.w == 1u;
 
@@ -784,10 +800,10 @@ size_t grid_work_bytes(const GridGeom &g, uint32_t max_points, int precision)
     b += align_up(sizeof(uint32_t) * g.nb, 256);           // counts
     b += align_up(sizeof(uint32_t) * g.nb, 256);           // cursor
     b += align_up(sizeof(uint32_t) * (g.nb + 1), 256);     // offsets
-    b += align_up(sizeof(uint32_t) * 4 * nscan, 256);      // blocksum
+    b += align_up(sizeof(uint32_t) * kScanQ * nscan, 256); // blocksum
     b += align_up(sizeof(uint4) * task_cap, 256);          // tasks
     b += align_up(sizeof(uint32_t) * split_cap, 256);      // split_list
-    b += align_up(sizeof(uint32_t) * 4, 256);              // meta
+    b += align_up(sizeof(uint32_t) * 8, 256);              // meta
     b += align_up(rs * rec_cap, 256);                      // records
     b += align_up(sizeof(uint32_t) * (size_t)g.planes * g.tiles_v, 256);  // band
     return b;
@@ -808,10 +824,10 @@ void grid_work_carve(const GridGeom &g, uint32_t max_points, int precision, void
     w->counts = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * g.nb));
     w->cursor = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * g.nb));
     w->offsets = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * (g.nb + 1)));
-    w->blocksum = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * 4 * nscan));
+    w->blocksum = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * kScanQ * nscan));
     w->tasks = reinterpret_cast<uint4 *>(take(sizeof(uint4) * task_cap));
     w->split_list = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * split_cap));
-    w->meta = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * 4));
+    w->meta = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * 8));
     w->records = take(rs * rec_cap);
     w->band = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * (size_t)g.planes * g.tiles_v));
     w->rec_cap = rec_cap;
@@ -846,7 +862,7 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
                        w.offsets, w.meta);
     hipLaunchKernelGGL(k_scan_down, dim3(w.nscan), dim3(kThreads), 0, s, g, w.counts,
                        w.blocksum, w.nscan, w.offsets, w.cursor, w.tasks, w.task_cap,
-                       w.split_list, w.split_cap, zero);
+                       w.split_list, w.split_cap, w.meta, zero);
     tm->end(RICK_STAGE_SCAN, s);
     tm->begin(RICK_STAGE_SCATTER, s);
     if (n > 0) {

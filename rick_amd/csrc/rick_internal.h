@@ -63,10 +63,12 @@ struct GridWork {
     uint32_t *counts;       // [nb]  records per bucket
     uint32_t *cursor;       // [nb]  scatter cursor (starts at the bucket offset)
     uint32_t *offsets;      // [nb+1] exclusive scan of counts
-    uint32_t *blocksum;     // [4*nscan] per scan block: records, tasks, split, non-empty
+    uint32_t *blocksum;     // [5*nscan] per scan block: records, accumulation tasks, split,
+                            //   non-empty, zero tasks
     uint4 *tasks;           // [task_cap] (bucket, first record, end record, split)
     uint32_t *split_list;   // [split_cap] buckets handled by more than one task
-    uint32_t *meta;         // [4]: total records, tasks, split buckets, non-empty buckets
+    uint32_t *meta;         // [8]: records, tasks (all), split buckets, non-empty buckets,
+                            //   accumulation tasks (stored before the zero tasks)
     void *records;          // [rec_cap] sorted row records
     uint32_t *band;         // [planes*tiles_v] tile rows holding an occupied tile
     uint32_t rec_cap, split_cap, task_cap, nscan;
