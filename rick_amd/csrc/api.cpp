@@ -168,6 +168,8 @@ int rick_plan_create(rick_plan *out, const rick_plan_desc *desc)
     g.nb = (uint32_t)nb;
     g.rmax = d.rows_per_task > 0 ? (uint32_t)d.rows_per_task : 4096u;
     g.ntable = kTableOversample * d.w_support;
+    // ~64-128 KB of grid per zero task measured best (LOFAR-like grid).
+    g.zrun = d.precision == RICK_FP64 ? 2 : 4;
     if ((uint64_t)d.max_points * 4ull >= (1ull << 32))
         return set_error(RICK_ERR_INVALID, "max_points %u too large (limit 2^30)", d.max_points);
 

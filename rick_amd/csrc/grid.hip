@@ -62,7 +62,6 @@ constexpr int kBinHashBits = 11;
 constexpr int kBinSlots = 1 << kBinHashBits;
 constexpr int kMaxProbe = 32;
 constexpr uint32_t kZeroTask = 2u;            // task.w of a zero task
-constexpr int kZeroRun = 1;                   // empty buckets per zero task (at most)
 constexpr int kTileThreads = 512;
 constexpr int kPadU = 8;  // LDS row padding (elements): rows shift by 8 banks
 
@@ -201,9 +200,9 @@ __device__ __forceinline__ uint32_t ntasks_of(uint32_t c, uint32_t rmax)
 
 // Tasks of a scan thread's 4 consecutive buckets (nv of them exist): those of
 // the non-empty ones, plus, when the grid is overwritten (zero != 0), one zero
-// task per kZeroRun consecutive empty ones.
+// task per g.zrun consecutive empty ones.
 __device__ __forceinline__ void group_tasks(const uint32_t *c, int nv, uint32_t rmax, int zero,
-                                            uint32_t &real, uint32_t &zeros)
+                                            int zrun, uint32_t &real, uint32_t &zeros)
 {
     real = zeros = 0;
     int run = 0;  // length of the current run of empty buckets
@@ -212,7 +211,7 @@ __device__ __forceinline__ void group_tasks(const uint32_t *c, int nv, uint32_t 
             real += ntasks_of(c[k], rmax);
             run = 0;
         } else {
-            if (zero && run % kZeroRun == 0) ++zeros;
+            if (zero && run % zrun == 0) ++zeros;
             ++run;
         }
     }
@@ -236,7 +235,7 @@ k_scan_reduce(GridGeom g, const uint32_t *__restrict__ counts, uint32_t *__restr
         }
     }
     uint32_t tr, tz;
-    group_tasks(c, nv, g.rmax, zero, tr, tz);
+    group_tasks(c, nv, g.rmax, zero, g.zrun, tr, tz);
     s[0][threadIdx.x] = r;
     s[1][threadIdx.x] = tr;
     s[2][threadIdx.x] = sp;
@@ -319,7 +318,7 @@ k_scan_down(GridGeom g, const uint32_t *__restrict__ counts,
             v[2] += c[k] > g.rmax ? 1u : 0u;
         }
     }
-    group_tasks(c, nv, g.rmax, zero, v[1], v[3]);
+    group_tasks(c, nv, g.rmax, zero, g.zrun, v[1], v[3]);
 #pragma unroll
     for (int a = 0; a < 4; ++a) s[a][threadIdx.x] = v[a];
     __syncthreads();
@@ -344,10 +343,10 @@ k_scan_down(GridGeom g, const uint32_t *__restrict__ counts,
         offsets[b] = ro;
         cursor[b] = ro;
         if (c[k] == 0) {
-            // Every kZeroRun-th bucket of a run of empty ones starts a zero task.
-            if (zero && run % kZeroRun == 0) {
+            // Every zrun-th bucket of a run of empty ones starts a zero task.
+            if (zero && run % g.zrun == 0) {
                 uint32_t len = 1;
-                while (len < (uint32_t)kZeroRun && k + (int)len < nv && c[k + len] == 0) ++len;
+                while (len < (uint32_t)g.zrun && k + (int)len < nv && c[k + len] == 0) ++len;
                 if (zo < task_cap) tasks[zo] = make_uint4(b, len, 0u, kZeroTask);
                 ++zo;
             }

@@ -56,6 +56,7 @@ struct GridGeom {
     uint32_t rmax;          // rows per task
     int ntable;
     double cmax;            // largest per-cell convolution weight (a bound)
+    int zrun;               // empty buckets cleared per zero task (1..4)
 };
 
 // Workspace of the binned gridder, carved out of one device allocation.
