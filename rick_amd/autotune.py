@@ -7,6 +7,7 @@ times the candidates on the actual workload and caches the fastest per
 workload signature in a JSON file (rick_amd/tuning.json by default).
 
     python -m rick_amd.autotune --config lofar --precision fp32
+    python -m rick_amd.autotune --batch 96 --precision fp32   # the batched job mix
 """
 from __future__ import annotations
 
@@ -79,17 +80,46 @@ def store(key: str, best: dict, path: Path = CACHE) -> None:
     os.replace(tmp, path)
 
 
+def tune_batch(n_jobs, precision, out, steps=2, log=print, seed=3):
+    """Tune every distinct signature of the batched-imaging job mix
+    (rick_amd.batch.job_configs) and store each best shape."""
+    import torch
+
+    from . import synth
+    from .batch import job_configs
+    seen = {}
+    for cfg in job_configs(n_jobs, seed):
+        key = signature(cfg.grid, cfg.planes, cfg.support, precision, cfg.n_rows)
+        if key in seen:
+            continue
+        ds = synth.make_dataset(cfg, device="cuda:0")
+        r = tune_grid(ds, precision, steps=steps, log=lambda m: None)
+        store(key, r["best"], Path(out))
+        seen[key] = r["best"]
+        log(f"[tune] {key}: tile {r['best']['tile']} rows/task {r['best']['rows_per_task']} "
+            f"{r['best']['ms']:.3f} ms")
+        del ds
+        torch.cuda.empty_cache()
+    return seen
+
+
 def main():
     from . import FP32, FP64, synth
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", default="lofar")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp32")
     ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=0,
+                    help="tune the signatures of this many batched-imaging jobs instead")
     ap.add_argument("--out", default=str(CACHE))
     args = ap.parse_args()
+    prec = FP32 if args.precision == "fp32" else FP64
+    if args.batch:
+        best = tune_batch(args.batch, prec, args.out, steps=max(1, args.steps - 1))
+        print(json.dumps({"batch_jobs": args.batch, "signatures": len(best)}))
+        return
     cfg = synth.CONFIGS[args.config]
     ds = synth.make_dataset(cfg, device="cuda:0")
-    prec = FP32 if args.precision == "fp32" else FP64
     r = tune_grid(ds, prec, steps=args.steps)
     key = signature(cfg.grid, cfg.planes, cfg.support, prec, cfg.n_rows)
     store(key, r["best"], Path(args.out))

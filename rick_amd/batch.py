@@ -33,8 +33,12 @@ BASELINES_64 = 64 * 63 // 2
 def job_configs(n_jobs: int = 96, seed: int = 3, vis_lo: float = 5e4, vis_hi: float = 8e6):
     """Job mix: grid in {256..2048}, 4-16 w-planes, visibility counts
     log-uniform in [vis_lo, vis_hi] from Earth-rotation tracks of 64 antennas
-    (2,016 baselines) at declinations 0..80 deg, Gaussian support 7 or
-    Kaiser-Bessel support 9."""
+    (2,016 baselines) at 150 MHz and declinations 0..80 deg, Gaussian support 7
+    or Kaiser-Bessel support 9.  The array is a 1.5 km-radius disk scaled with
+    the grid (x grid/512 above 512): the phase stage's pixel angle reaches
+    (N-1) * resolution with resolution = 1/max|uv| (phase_correction.cu:204-
+    209), so a grid much wider than the longest baseline in wavelengths would
+    put most pixels beyond l^2 + m^2 = 1 (NaN, as in the reference)."""
     rng = np.random.default_rng(seed)
     out = []
     for j in range(n_jobs):
@@ -44,7 +48,8 @@ def job_configs(n_jobs: int = 96, seed: int = 3, vis_lo: float = 5e4, vis_hi: fl
         kernel, support = KERNELS[int(rng.integers(0, 2))]
         n_times = max(1, int(round(vis / BASELINES_64)))
         dec = float(rng.uniform(0.0, 80.0))
-        out.append(synth.ObsConfig(f"job{j}", "disk", 64, 1500.0, 45.0, n_times, -3.0, 3.0, dec,
+        radius = 1500.0 * max(1.0, grid / 512)
+        out.append(synth.ObsConfig(f"job{j}", "disk", 64, radius, 45.0, n_times, -3.0, 3.0, dec,
                                    150e6, 150e6, 1, 1, 5, grid, planes, kernel, support,
                                    seed=seed * 1000 + j))
     return out

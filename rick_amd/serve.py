@@ -14,7 +14,7 @@ Faults handled without disturbing other jobs:
   * a job whose launch fails (bad arguments) completes with an error;
   * rows with NaN or out-of-range uvw are dropped by the gridder itself.
 
-    python -m rick_amd.serve --duration 20 --rate 8      # prints one JSON line
+    python -m rick_amd.serve --duration 20 --rate 400    # prints one JSON line
 """
 from __future__ import annotations
 
@@ -240,7 +240,7 @@ def workload(n_small=24, n_lofar=2, seed=5, lofar_vis=20_000_000):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--duration", type=float, default=20.0, help="seconds of arrivals")
-    ap.add_argument("--rate", type=float, default=8.0, help="requests per second")
+    ap.add_argument("--rate", type=float, default=400.0, help="requests per second")
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--budget-gb", type=float, default=256.0)
     args = ap.parse_args()
