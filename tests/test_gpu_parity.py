@@ -73,14 +73,16 @@ def test_grid_tiles_and_split_buckets(tile, rows_per_task):
 
 
 @pytest.mark.parametrize("precision", [FP64, FP32])
-def test_grid_scattered_rows_overflow_block_tables(precision):
+@pytest.mark.parametrize("F,P", [(2, 1), (16, 1), (4, 2), (64, 1)])
+def test_grid_scattered_rows_overflow_block_tables(precision, F, P):
     """20,000 rows spread uniformly over 16 planes of 1024^2 in 32x32 tiles:
-    each 8,192-row block touches thousands of distinct buckets, more than its
-    hash table holds, so many records take the per-bucket overflow range;
-    also spans three row blocks and a partial last block.  Includes rows at
-    w = 1.0 (dropped) and NaN visibilities (skipped)."""
+    each row block touches thousands of distinct buckets, more than its hash
+    table holds, so many records take the direct-atomic path; also spans
+    several row blocks and a partial last block.  Includes rows at w = 1.0
+    (dropped) and NaN visibilities (skipped).  F*P in {2, 16, 8, 64} covers
+    the per-row and the cooperative (coalesced) visibility sums."""
     rng = np.random.default_rng(123)
-    n, F, P, nx, planes = 20000, 2, 1, 1024, 16
+    n, nx, planes = 20000, 1024, 16
     uu = rng.uniform(0.0, 1.0, n)
     vv = rng.uniform(0.0, 1.0, n)
     ww = rng.uniform(0.0, 1.0, n)
@@ -91,7 +93,7 @@ def test_grid_scattered_rows_overflow_block_tables(precision):
     wt = rng.uniform(0.5, 1.5, n * P).astype(np.float32)
     ref = O.wstack(O.KAISERBESSEL, planes, uu, vv, ww, vr, vi, wt, F, P, 1.0 / nx, 1.0 / planes,
                    7, nx, nx, threads=8)
-    print(f"case scattered precision={precision}", flush=True)
+    print(f"case scattered precision={precision} F={F} P={P}", flush=True)
     plan = Plan(nx, nx, planes, kernel=rick_amd.KAISERBESSEL, support=7, precision=precision,
                 max_points=n, tile=(32, 32))
     ins = [dev(a) for a in (uu, vv, ww, vr, vi, wt)]
