@@ -141,8 +141,12 @@ def test_grid_accumulate_and_empty_input():
     assert not host(g).any()
 
 
-def test_grid_fp32_within_tolerance():
-    z = load_golden("wstack_kb_s9.npz")
+@pytest.mark.parametrize("name", ["wstack_kb_s9", "wstack_gauss_s7", "wstack_gausshp_s9"])
+def test_grid_fp32_within_tolerance(name):
+    """FP32 path (fixed-point tile accumulation) vs the reference's FP64 grid,
+    for every convolution kernel (the hi-precision Gaussian bounds its cell
+    weights by its norm)."""
+    z = load_golden(f"{name}.npz")
     plan = grid_plan(z, precision=FP32)
     got = host(plan.grid(*grid_inputs(z), int(z["F"]), int(z["P"]), out=nan_grid(plan)))
     assert got.dtype == np.float32
