@@ -178,6 +178,108 @@ int rick_plan_image(rick_plan plan, unsigned int num_points, unsigned int freq_p
                     const float *weight, void *grid, double *image_real,
                     double *image_imag, void *stream);
 
+/* ------------------------------------------------------------------------- */
+/* 3. Data on either side of the chain: the binMS input format, device-side  */
+/*    NORMALIZE_UVW and sector binning in front of wstack(), and the image   */
+/*    and cube writers behind phase_correction().                            */
+
+/* The twelve values of meta.txt that the reference parses (struct meta,
+ * allvars.h; read by readMetaData, init.c:300-333). */
+typedef struct {
+    unsigned int nmeasures;      /* rows                                   */
+    unsigned long nvis;          /* rows * freq_per_chan * polarisations   */
+    unsigned int freq_per_chan;
+    unsigned int polarisations;
+    unsigned int ntimes;
+    double dt, thours;
+    unsigned int baselines;
+    double uvmin, uvmax, wmin, wmax;
+} rick_binms_meta;
+
+/* Replaces readMetaData() (init.c:300-333): parses <dir>/meta.txt, the first
+ * twelve whitespace-separated values in the reference's order and formats
+ * (%u %lu %u %u %u %lf %lf %u %lf %lf %lf %lf); later lines are ignored.
+ * Fails (instead of exit(1)) when the file is missing or short. */
+int rick_binms_read_meta(const char *dir, rick_binms_meta *meta);
+
+/* Replaces the row split of metaData_calculation() (init.c:335-372): the last
+ * `drop` rows are left out (the reference drops 1000), the rest is split into
+ * `size` equal parts, the remainder going to the last rank. */
+int rick_binms_partition(const rick_binms_meta *meta, int rank, int size, unsigned int drop,
+                         unsigned long *startrow, unsigned int *nrows);
+
+/* Replaces readData() (init.c:424-490): reads rows [startrow, startrow+nrows)
+ * of ucoord.bin, vcoord.bin, wcoord.bin (f64), weights.bin (f32, P per row),
+ * visibilities_real.bin and visibilities_img.bin (f32, F*P per row) under
+ * dir.  device < 0: the six arrays are host memory; device >= 0: they are
+ * device memory on that device and each file is streamed through two pinned
+ * staging buffers.  Short files are an error. */
+int rick_binms_read(const char *dir, unsigned int freq_per_chan, unsigned int polarizations,
+                    unsigned long startrow, unsigned int nrows, double *uu, double *vv, double *ww,
+                    float *weight, float *vis_real, float *vis_img, int device);
+
+/* Writes a binMS directory (host arrays) in the layout of the reference's
+ * converter, scripts/create_binMS.py:210-245: the six raw arrays and a
+ * meta.txt of 15 lines (the twelve values above, then the weight array's
+ * shape: rows, polarisations, 1.0).  meta->nvis must be rows*F*P. */
+int rick_binms_write(const char *dir, const rick_binms_meta *meta, const double *uu,
+                     const double *vv, const double *ww, const float *weight,
+                     const float *vis_real, const float *vis_img);
+
+/* Replaces the raw image output of WRITE_IMAGE (fourier_transform.c:280-289):
+ * each file is created ("wb") and the xaxis*yaxis doubles are written at byte
+ * rank*xaxis*yaxis*8.  Host arrays. */
+int rick_write_image(const char *real_path, const char *imag_path, const double *image_real,
+                     const double *image_imag, int xaxis, int yaxis, int rank);
+
+/* Replaces the WRITE_DATA cube writers write_gridded_data() (gridding.c:
+ * 210-282) and write_fftw_data() (fourier_transform.c:134-201) for one rank:
+ * the real and imaginary parts of a grid (layout above; FP64 or FP32, written
+ * as doubles) go to two files, plane after plane.  device < 0: grid is host
+ * memory; device >= 0: device memory on that device, de-interleaved there. */
+int rick_write_cube(const char *real_path, const char *imag_path, const void *grid, int precision,
+                    int num_w_planes, int grid_size_x, int grid_size_y, int device);
+
+/* NORMALIZE_UVW (gridding.c:26-82) on device arrays, in place, stream-
+ * ordered: u,v -> (x + maxg)/(2*maxg) with maxg the largest |u|,|v| extremum
+ * plus 0.1 %, w -> (w - wmin)/(wmax - wmin), extrema folded from 1e20 (min)
+ * and 0.0 (max) as the reference does.  extrema (device double[8], may be
+ * NULL) receives min u, v, w, max u, v, w, maxg, 0.  workspace: device
+ * memory of rick_normalize_uvw_workspace_bytes().  Bitwise the reference's
+ * results for NaN-free input; all-equal w gives NaN as in the reference. */
+size_t rick_normalize_uvw_workspace_bytes(void);
+int rick_normalize_uvw(unsigned int num_points, double *uu, double *vv, double *ww,
+                       double *extrema, void *workspace, void *stream);
+
+/* Sector binning of initialize_array() (gridding.c:139-197) on the device:
+ * each row goes to slab binphi = (int)(v*nsectors) and to the slab above /
+ * below when within ((w_support-1)/2)*dx of its border.  Outputs (device):
+ * histo[nsectors+1] and start[nsectors+1] (the reference's histo_send and
+ * the offsets of each sector's list in rows), rows[] the concatenated lists,
+ * each in increasing row order as the reference fills them.  Synchronizes
+ * the stream; *total_entries receives the number of entries and *dropped the
+ * entries the reference would store out of bounds (binphi outside
+ * 0..nsectors), which are left out.  Fails if rows_capacity is too small
+ * (3*num_points always suffices).  nsectors <= 4095, num_points <= 2^30.
+ * workspace: rick_sector_workspace_bytes(num_points, nsectors) bytes. */
+size_t rick_sector_workspace_bytes(unsigned int num_points, int nsectors);
+int rick_sector_bin(unsigned int num_points, const double *vv, int nsectors, int yaxis, double dx,
+                    int w_support, unsigned int *histo, unsigned int *start, unsigned int *rows,
+                    unsigned int rows_capacity, void *workspace, void *stream,
+                    unsigned long *total_entries, unsigned int *dropped);
+
+/* Per-sector input assembly of gridding_cpu.c:90-138 on the device: output
+ * row i takes row list[nsec-1-i] (the reference walks the list backwards),
+ * v shifted by isector*dx*yaxis; weights (P per row) and visibilities (F*P
+ * per row) follow their row.  The result is what rank isector passes to
+ * wstack().  List entries >= num_points are skipped.  Stream-ordered. */
+int rick_sector_gather(const unsigned int *list, unsigned int nsec, int isector, int yaxis,
+                       double dx, unsigned int num_points, unsigned int freq_per_chan,
+                       unsigned int polarizations, const double *uu, const double *vv,
+                       const double *ww, const float *weight, const float *vis_real,
+                       const float *vis_img, double *uus, double *vvs, double *wws,
+                       float *weightss, float *visreals, float *visimgs, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

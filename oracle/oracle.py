@@ -56,6 +56,14 @@ def lib() -> ctypes.CDLL:
                                               _c_int, _c_int]
         L.oracle_normalize_uvw.restype = None
         L.oracle_normalize_uvw.argtypes = [ctypes.c_size_t, _dp, _dp, _dp]
+        _up = ctypes.POINTER(_c_uint)
+        L.oracle_sector_bin.restype = _c_uint
+        L.oracle_sector_bin.argtypes = [ctypes.c_size_t, _dp, _c_int, _c_int, _c_double, _c_int,
+                                        _up, _up, _up]
+        L.oracle_sector_gather.restype = None
+        L.oracle_sector_gather.argtypes = [_up, _c_uint, _c_int, _c_int, _c_double, _c_uint,
+                                           _c_uint, _dp, _dp, _dp, _fp, _fp, _fp, _dp, _dp, _dp,
+                                           _fp, _fp, _fp]
         _lib = L
     return _lib
 
@@ -113,6 +121,60 @@ def normalize_uvw(uu, vv, ww):
     uu, vv, ww = (np.array(a, dtype=np.float64, copy=True) for a in (uu, vv, ww))
     lib().oracle_normalize_uvw(uu.size, _ptr(uu, _dp), _ptr(vv, _dp), _ptr(ww, _dp))
     return uu, vv, ww
+
+
+def sector_bin(vv, nsectors, yaxis, dx, w_support):
+    """initialize_array (gridding.c:139-197): (histo, start, rows, dropped)."""
+    vv = _c(vv, np.float64)
+    histo = np.zeros(nsectors + 1, dtype=np.uint32)
+    start = np.zeros(nsectors + 1, dtype=np.uint32)
+    rows = np.zeros(max(1, 3 * vv.size), dtype=np.uint32)
+    up = ctypes.POINTER(_c_uint)
+    bad = lib().oracle_sector_bin(vv.size, _ptr(vv, _dp), nsectors, yaxis, dx, w_support,
+                                  _ptr(histo, up), _ptr(start, up), _ptr(rows, up))
+    return histo, start, rows[:int(histo.sum())], int(bad)
+
+
+def sector_gather(list_rows, isector, yaxis, dx, freq_per_chan, polarizations, uu, vv, ww,
+                  weight, vis_real, vis_img):
+    """gridding_cpu.c:90-138 for one sector: dict of the arrays passed to wstack."""
+    lst = _c(list_rows, np.uint32)
+    n = lst.size
+    F, P = freq_per_chan, polarizations
+    out = {"uu": np.zeros(n), "vv": np.zeros(n), "ww": np.zeros(n),
+           "weight": np.zeros(n * P, np.float32), "vis_real": np.zeros(n * F * P, np.float32),
+           "vis_img": np.zeros(n * F * P, np.float32)}
+    ins = [_c(uu, np.float64), _c(vv, np.float64), _c(ww, np.float64), _c(weight, np.float32),
+           _c(vis_real, np.float32), _c(vis_img, np.float32)]
+    lib().oracle_sector_gather(
+        _ptr(lst, ctypes.POINTER(_c_uint)), n, isector, yaxis, dx, F, P,
+        *(_ptr(a, _dp if a.dtype == np.float64 else _fp) for a in ins),
+        *(_ptr(out[k], _dp if out[k].dtype == np.float64 else _fp) for k in
+          ("uu", "vv", "ww", "weight", "vis_real", "vis_img")))
+    return out
+
+
+def read_binms(path, freq_per_chan, polarizations, startrow, nrows):
+    """readData (init.c:424-490): the six arrays of rows [startrow, startrow+nrows)
+    read at the reference's byte offsets."""
+    F, P = freq_per_chan, polarizations
+    spec = [("ucoord.bin", np.float64, 1), ("vcoord.bin", np.float64, 1),
+            ("wcoord.bin", np.float64, 1), ("weights.bin", np.float32, P),
+            ("visibilities_real.bin", np.float32, F * P),
+            ("visibilities_img.bin", np.float32, F * P)]
+    out = []
+    for name, dt, per in spec:
+        item = np.dtype(dt).itemsize
+        out.append(np.fromfile(os.path.join(str(path), name), dtype=dt, count=nrows * per,
+                               offset=startrow * per * item))
+    return dict(zip(("uu", "vv", "ww", "weight", "vis_real", "vis_img"), out))
+
+
+def partition(nmeasures, rank, size, drop=1000):
+    """metaData_calculation (init.c:335-372): (startrow, nrows) of rank."""
+    total = nmeasures - drop
+    per, rem = divmod(total, size)
+    return rank * per, per + (rem if rank == size - 1 else 0)
 
 
 # ---------------------------------------------------------------------------

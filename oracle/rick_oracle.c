@@ -358,3 +358,81 @@ void oracle_normalize_uvw(size_t n, double *uu, double *vv, double *ww)
         ww[i] = (ww[i] - minw) / (maxw - minw);
     }
 }
+
+/* Slots of one row, in the order the reference tests them (gridding.c:
+ * 156-167); -1 marks no entry.  Returns the out-of-bounds entries skipped. */
+static unsigned int sector_slots(double vvh, int nsectors, int yaxis, double dx, double supporth,
+                                 int s[3])
+{
+    s[0] = s[1] = s[2] = -1;
+    const double b = vvh * nsectors;
+    if (!(b > -1.0 && b < (double)nsectors + 1.0)) return 1;
+    const int binphi = (int)b;
+    const double updist = (double)((binphi + 1) * yaxis) * dx - vvh;
+    const double downdist = vvh - (double)(binphi * yaxis) * dx;
+    unsigned int bad = 0;
+    s[0] = binphi;
+    if (updist < supporth && updist >= 0.0) {
+        if (binphi + 1 <= nsectors) s[1] = binphi + 1;
+        else bad++;
+    }
+    if (downdist < supporth && binphi > 0 && downdist >= 0.0) s[2] = binphi - 1;
+    return bad;
+}
+
+unsigned int oracle_sector_bin(size_t n, const double *vv, int nsectors, int yaxis, double dx,
+                               int w_support, unsigned int *histo, unsigned int *start,
+                               unsigned int *rows)
+{
+    const double supporth = (double)((w_support - 1) / 2) * dx;  /* init.c:13 */
+    unsigned int bad = 0;
+    int s[3];
+    if (nsectors < 1) return 0;
+    for (int k = 0; k <= nsectors; k++) histo[k] = 0;
+    /* First pass: histo_send (gridding.c:150-168). */
+    for (size_t i = 0; i < n; i++) {
+        bad += sector_slots(vv[i], nsectors, yaxis, dx, supporth, s);
+        for (int j = 0; j < 3; j++)
+            if (s[j] >= 0) histo[s[j]]++;
+    }
+    unsigned int acc = 0;
+    for (int k = 0; k <= nsectors; k++) {
+        start[k] = acc;
+        acc += histo[k];
+    }
+    /* Second pass: sectorarray[s][counter[s]++] = row (gridding.c:178-190). */
+    unsigned int *counter = calloc((size_t)(unsigned)nsectors + 1u, sizeof *counter);
+    for (size_t i = 0; i < n; i++) {
+        sector_slots(vv[i], nsectors, yaxis, dx, supporth, s);
+        for (int j = 0; j < 3; j++)
+            if (s[j] >= 0) rows[start[s[j]] + counter[s[j]]++] = (unsigned int)i;
+    }
+    free(counter);
+    return bad;
+}
+
+void oracle_sector_gather(const unsigned int *list, unsigned int nsec, int isector, int yaxis,
+                          double dx, unsigned int freq_per_chan, unsigned int polarizations,
+                          const double *uu, const double *vv, const double *ww,
+                          const float *weight, const float *vis_real, const float *vis_img,
+                          double *uus, double *vvs, double *wws, float *weightss,
+                          float *visreals, float *visimgs)
+{
+    const double shift = (double)(dx * yaxis);     /* gridding_cpu.c:24 */
+    const unsigned int isec = (unsigned int)isector;
+    size_t icount = 0, ip = 0, inu = 0;
+    for (long iphi = (long)nsec - 1; iphi >= 0; iphi--) {
+        const size_t il = list[iphi];
+        uus[icount] = uu[il];
+        vvs[icount] = vv[il] - isec * shift;
+        wws[icount] = ww[il];
+        for (unsigned int p = 0; p < polarizations; p++)
+            weightss[ip++] = weight[il * polarizations + p];
+        for (unsigned int f = 0; f < polarizations * freq_per_chan; f++) {
+            visreals[inu] = vis_real[il * polarizations * freq_per_chan + f];
+            visimgs[inu] = vis_img[il * polarizations * freq_per_chan + f];
+            inu++;
+        }
+        icount++;
+    }
+}

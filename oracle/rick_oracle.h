@@ -71,6 +71,26 @@ void oracle_phase_correction(const double *gridss, double *image_real,
 /* NORMALIZE_UVW block of gridding() (gridding.c:21-83), in place. */
 void oracle_normalize_uvw(size_t n, double *uu, double *vv, double *ww);
 
+/* initialize_array (gridding.c:139-197): the two sequential passes over the
+ * rows that count (histo[nsectors+1]) and fill the per-sector row lists,
+ * concatenated in rows[] with sector s starting at start[s].  Entries the
+ * reference would store out of bounds (binphi outside 0..nsectors, an upper
+ * neighbour beyond slot nsectors) are left out and counted; returns that
+ * count.  rows must hold 3*n entries. */
+unsigned int oracle_sector_bin(size_t n, const double *vv, int nsectors, int yaxis, double dx,
+                               int w_support, unsigned int *histo, unsigned int *start,
+                               unsigned int *rows);
+
+/* The per-sector assembly loop of gridding_cpu.c:90-138: the list walked
+ * backwards, v shifted by isector*dx*yaxis, weights and visibilities copied
+ * with their rows. */
+void oracle_sector_gather(const unsigned int *list, unsigned int nsec, int isector, int yaxis,
+                          double dx, unsigned int freq_per_chan, unsigned int polarizations,
+                          const double *uu, const double *vv, const double *ww,
+                          const float *weight, const float *vis_real, const float *vis_img,
+                          double *uus, double *vvs, double *wws, float *weightss,
+                          float *visreals, float *visimgs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,6 +33,20 @@ class PlanDesc(ctypes.Structure):
     ]
 
 
+class BinmsMeta(ctypes.Structure):
+    """rick_binms_meta (include/rick.h): the twelve parsed values of meta.txt."""
+    _fields_ = [
+        ("nmeasures", _c_uint), ("nvis", ctypes.c_ulong), ("freq_per_chan", _c_uint),
+        ("polarisations", _c_uint), ("ntimes", _c_uint), ("dt", _c_double),
+        ("thours", _c_double), ("baselines", _c_uint), ("uvmin", _c_double),
+        ("uvmax", _c_double), ("wmin", _c_double), ("wmax", _c_double),
+    ]
+
+
+_cp = ctypes.c_char_p
+_up = ctypes.POINTER(_c_uint)
+_ulp = ctypes.POINTER(ctypes.c_ulong)
+
 # name -> (restype, argtypes); every symbol declared in include/rick.h.
 SIGNATURES = {
     "wstack": (None, [_c_int, _c_uint, _c_uint, _c_uint, _dp, _dp, _dp, _fp, _fp, _fp, _c_double,
@@ -61,6 +75,22 @@ SIGNATURES = {
     "rick_plan_grid_stats": (_c_int, [_vp, ctypes.POINTER(_c_uint), _c_int]),
     "rick_plan_image": (_c_int, [_vp, _c_uint, _c_uint, _c_uint, _vp, _vp, _vp, _vp, _vp, _vp,
                                  _vp, _vp, _vp, _vp]),
+    "rick_binms_read_meta": (_c_int, [_cp, ctypes.POINTER(BinmsMeta)]),
+    "rick_binms_partition": (_c_int, [ctypes.POINTER(BinmsMeta), _c_int, _c_int, _c_uint, _ulp,
+                                      _up]),
+    "rick_binms_read": (_c_int, [_cp, _c_uint, _c_uint, ctypes.c_ulong, _c_uint, _vp, _vp, _vp,
+                                 _vp, _vp, _vp, _c_int]),
+    "rick_binms_write": (_c_int, [_cp, ctypes.POINTER(BinmsMeta), _vp, _vp, _vp, _vp, _vp, _vp]),
+    "rick_write_image": (_c_int, [_cp, _cp, _vp, _vp, _c_int, _c_int, _c_int]),
+    "rick_write_cube": (_c_int, [_cp, _cp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int]),
+    "rick_normalize_uvw_workspace_bytes": (_c_size, []),
+    "rick_normalize_uvw": (_c_int, [_c_uint, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "rick_sector_workspace_bytes": (_c_size, [_c_uint, _c_int]),
+    "rick_sector_bin": (_c_int, [_c_uint, _vp, _c_int, _c_int, _c_double, _c_int, _vp, _vp, _vp,
+                                 _c_uint, _vp, _vp, _ulp, _up]),
+    "rick_sector_gather": (_c_int, [_vp, _c_uint, _c_int, _c_int, _c_double, _c_uint, _c_uint,
+                                    _c_uint, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                    _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -42,6 +42,23 @@ void clear_error()
     g_msg[0] = '\0';
 }
 
+}  // namespace
+
+namespace rick {
+int set_status(int code, const char *fmt, ...)
+{
+    g_status = code;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_msg, sizeof g_msg, fmt, ap);
+    va_end(ap);
+    return code;
+}
+void clear_status() { clear_error(); }
+}  // namespace rick
+
+namespace {
+
 #define RICK_HIP(call)                                                                        \
     do {                                                                                      \
         hipError_t e_ = (call);                                                               \

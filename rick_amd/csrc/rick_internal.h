@@ -11,6 +11,16 @@
 
 namespace rick {
 
+// Status reporting shared by the C ABI entry points (rick_last_status /
+// rick_last_error): set_status records code and message and returns code.
+int set_status(int code, const char *fmt, ...);
+void clear_status();
+
+// De-interleaves n complex grid elements (FP64 or FP32) into two double
+// arrays on stream s.
+hipError_t launch_split_complex(const void *grid, int precision, size_t n, double *re, double *im,
+                                hipStream_t s);
+
 // Table oversampling of the reference (w-stacking.cu:253).
 constexpr int kOversample = 5;
 // Truncated pi literal of the reference (w-stacking.h:6).
