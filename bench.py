@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--rows-per-task", type=int, default=0)
     ap.add_argument("--tune", action="store_true",
                     help="autotune tile shape and rows/task first (rick_amd.autotune)")
+    ap.add_argument("--no-prep", action="store_true",
+                    help="skip the device preparation rates (normalize, binning, gather)")
     ap.add_argument("--no-dropin", action="store_true",
                     help="skip the host-pointer wstack() drop-in timing (PCIe-inclusive)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -185,6 +187,63 @@ def dropin_rate(ds, cfg, torch):
                     "the FP64 grid through pinned chunks, += into the host grid"}
 
 
+def prep_rates(ds, cfg, torch, steps=5):
+    """Device preparation in front of the gridder on the workload's rows:
+    NORMALIZE_UVW (in place, on copies), sector binning for one GPU and for
+    an 8-slab split, and the gather of one GPU's sector.  Timed with events on
+    the current stream; algorithmic bytes per call are stated with each."""
+    from rick_amd import prep
+    rows, F, P = cfg.n_rows, cfg.n_chan, cfg.n_pol
+    dx = 1.0 / cfg.grid
+    out = {}
+
+    def clock(fn, reset=None):
+        ts = []
+        for i in range(steps + 1):
+            if reset:
+                reset()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            b.synchronize()
+            if i:
+                ts.append(a.elapsed_time(b))
+        return sum(ts) / len(ts)
+
+    u, v, w = ds.uu.clone(), ds.vv.clone(), ds.ww.clone()
+    ws = torch.empty(1 << 20, dtype=torch.uint8, device="cuda:0")
+
+    def reset():
+        u.copy_(ds.uu), v.copy_(ds.vv), w.copy_(ds.ww)
+
+    ms = clock(lambda: prep.normalize_uvw(u, v, w, ws), reset)
+    alg = rows * 72            # extrema pass reads u,v,w; apply reads and writes them
+    out["normalize_uvw"] = {"ms": ms, "alg_bytes": alg, "GBs": alg / ms / 1e6,
+                            "frac_of_peak": alg / ms / 1e6 / HBM_PEAK_GBS}
+    for S in (1, 8):
+        bins = {}
+
+        def run():
+            bins["b"] = prep.sector_bin(ds.vv, S, cfg.grid // S, dx, cfg.support)
+        ms = clock(run)
+        b = bins["b"]
+        alg = rows * 16 + b.total * 4        # v read by the count and fill passes, lists written
+        out[f"sector_bin_{S}"] = {"ms": ms, "entries": b.total, "alg_bytes": alg,
+                                  "GBs": alg / ms / 1e6,
+                                  "note": "includes the stream sync and the total's readback"}
+    b = prep.sector_bin(ds.vv, 1, cfg.grid, dx, cfg.support)
+    lst = b.sector(0)
+    ms = clock(lambda: prep.sector_gather(lst, 0, cfg.grid, dx, F, P, ds.uu, ds.vv, ds.ww,
+                                          ds.weight, ds.vis_real, ds.vis_img))
+    per_row = 24 + 4 * P + 8 * F * P
+    alg = rows * (4 + 2 * per_row)           # list read, row data read and written
+    out["sector_gather"] = {"ms": ms, "alg_bytes": alg, "GBs": alg / ms / 1e6,
+                            "frac_of_peak": alg / ms / 1e6 / HBM_PEAK_GBS,
+                            "note": "includes the output allocations of the Python wrapper"}
+    return out
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -309,6 +368,9 @@ def main():
             "image_rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
             "image_peak_rel": float(np.abs(a - b).max() / np.abs(b).max()),
         }
+    if not args.no_prep:
+        log("device preparation: normalize, sector binning, gather")
+        out["prep"] = prep_rates(ds, cfg, torch)
     if not args.no_dropin and not args.only:
         log("drop-in wstack() on host arrays")
         out["dropin_pcie_inclusive"] = dropin_rate(ds, cfg, torch)

@@ -32,6 +32,7 @@ constexpr int kExtBlocks = 1024;        // blocks of the extrema pass (max)
 constexpr int kSecRows = 4096;          // rows per sector-binning block
 constexpr int kSecRowsPerWave = kSecRows / (kPrepThreads / 64);
 constexpr int kMaxSlots = 4096;         // nsectors + 1 <= kMaxSlots
+constexpr int kAggregateSectors = 4;    // up to this many: wave-aggregated counting
 constexpr int kScanChunk = 4096;        // elements per scan block (16 per thread)
 constexpr int kScanTopThreads = 1024;
 constexpr int kMaxScanChunks = kScanTopThreads * 16;
@@ -105,10 +106,18 @@ k_uvw_extrema(uint32_t n, const double *__restrict__ uu, const double *__restric
     e.init();
     const size_t stride = (size_t)gridDim.x * kPrepThreads;
     for (size_t r = (size_t)blockIdx.x * kPrepThreads + threadIdx.x; r < n; r += stride) {
+        // This thread visits rows in increasing order, so a tie always
+        // goes to the row at hand.
         const double x[3] = {uu[r], vv[r], ww[r]};
         for (int q = 0; q < 3; ++q) {
-            take_min(e.v[q], e.i[q], x[q], (long long)r);
-            take_max(e.v[q + 3], e.i[q + 3], x[q], (long long)r);
+            if (x[q] <= e.v[q]) {
+                e.v[q] = x[q];
+                e.i[q] = (long long)r;
+            }
+            if (x[q] >= e.v[q + 3]) {
+                e.v[q + 3] = x[q];
+                e.i[q + 3] = (long long)r;
+            }
         }
     }
     e.block_reduce();
@@ -208,6 +217,29 @@ __device__ inline void row_sectors(const SectorGeom &g, double vvh, int s[3], ui
     if (downdist < g.supporth && binphi > 0 && downdist >= 0.0) s[2] = binphi - 1;
 }
 
+// Walks the distinct sectors among the wave's entries (s[0..2] per lane,
+// -1 = none), wave-uniformly: f(sec, mask, mine) runs once per sector with
+// the ballot of the lanes holding an entry for it.  Each round retires at
+// least the first pending entry of the lowest pending lane, so the loop ends.
+template <typename Fn>
+__device__ inline void for_each_sector(const int s[3], Fn &&f)
+{
+    int pending = (s[0] >= 0 ? 1 : 0) | (s[1] >= 0 ? 2 : 0) | (s[2] >= 0 ? 4 : 0);
+    for (;;) {
+        const unsigned long long act = __ballot(pending != 0);
+        if (act == 0) break;
+        const int leader = __ffsll((long long)act) - 1;
+        const int first = (pending & 1) ? s[0] : (pending & 2) ? s[1] : s[2];
+        const int sec = __builtin_amdgcn_readlane(first, leader);
+        int hit = 0;
+        for (int j = 0; j < 3; ++j)
+            if ((pending >> j & 1) && s[j] == sec) hit = 1 << j;
+        const unsigned long long m = __ballot(hit != 0);
+        f(sec, m, hit != 0);
+        pending &= ~hit;
+    }
+}
+
 __global__ void __launch_bounds__(kPrepThreads)
 k_sector_count(SectorGeom g, const double *__restrict__ vv, uint32_t *__restrict__ counts,
                uint32_t *__restrict__ bad_total)
@@ -219,14 +251,21 @@ k_sector_count(SectorGeom g, const double *__restrict__ vv, uint32_t *__restrict
     if (threadIdx.x == 0) bad_blk = 0;
     __syncthreads();
     uint32_t bad = 0;
+    const int lane = threadIdx.x & 63;
     const size_t r0 = (size_t)blockIdx.x * kSecRows;
     for (int k = 0; k < kSecRows / kPrepThreads; ++k) {
         const size_t r = r0 + (size_t)k * kPrepThreads + threadIdx.x;
-        if (r >= g.n) break;
-        int s[3];
-        row_sectors(g, vv[r], s, bad);
-        for (int j = 0; j < 3; ++j)
-            if (s[j] >= 0) atomicAdd(&hist[s[j]], 1u);
+        int s[3] = {-1, -1, -1};
+        if (r < g.n) row_sectors(g, vv[r], s, bad);
+        if (g.nsectors <= kAggregateSectors) {
+            // Few sectors: most lanes share one; one LDS atomic per sector.
+            for_each_sector(s, [&](int sec, unsigned long long m, bool) {
+                if (lane == 0) atomicAdd(&hist[sec], (uint32_t)__popcll(m));
+            });
+        } else {
+            for (int j = 0; j < 3; ++j)
+                if (s[j] >= 0) atomicAdd(&hist[s[j]], 1u);
+        }
     }
     if (bad) atomicAdd(&bad_blk, bad);
     __syncthreads();
@@ -331,14 +370,27 @@ k_sector_fill(SectorGeom g, const double *__restrict__ vv, const uint32_t *__res
     for (int i = threadIdx.x; i < waves * slots; i += kPrepThreads) cur[i] = 0;
     __syncthreads();
     const size_t r0 = (size_t)blockIdx.x * kSecRows + (size_t)wave * kSecRowsPerWave;
+    constexpr int kChunks = kSecRowsPerWave / 64;
+    double v[kChunks];   // this lane's rows, read once for both passes
     uint32_t bad = 0;
-    for (int c = 0; c < kSecRowsPerWave; c += 64) {
-        const size_t r = r0 + c + lane;
-        if (r < g.n) {
-            int s[3];
-            row_sectors(g, vv[r], s, bad);
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) {
+        const size_t r = r0 + c * 64 + lane;
+        v[c] = r < g.n ? vv[r] : 0.0;
+    }
+    uint32_t *my = cur + wave * slots;
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) {
+        const size_t r = r0 + c * 64 + lane;
+        int s[3] = {-1, -1, -1};
+        if (r < g.n) row_sectors(g, v[c], s, bad);
+        if (g.nsectors <= kAggregateSectors) {
+            for_each_sector(s, [&](int sec, unsigned long long m, bool) {
+                if (lane == 0) my[sec] += (uint32_t)__popcll(m);
+            });
+        } else {
             for (int j = 0; j < 3; ++j)
-                if (s[j] >= 0) atomicAdd(&cur[wave * slots + s[j]], 1u);
+                if (s[j] >= 0) atomicAdd(&my[s[j]], 1u);
         }
     }
     __syncthreads();
@@ -351,32 +403,22 @@ k_sector_fill(SectorGeom g, const double *__restrict__ vv, const uint32_t *__res
         }
     }
     __syncthreads();
-    uint32_t *my = cur + wave * slots;
-    for (int c = 0; c < kSecRowsPerWave; c += 64) {
-        const size_t r = r0 + c + lane;
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) {
+        const size_t r = r0 + c * 64 + lane;
         int s[3] = {-1, -1, -1};
-        if (r < g.n) row_sectors(g, vv[r], s, bad);
-        int pending = (s[0] >= 0 ? 1 : 0) | (s[1] >= 0 ? 2 : 0) | (s[2] >= 0 ? 4 : 0);
-        for (;;) {
-            const unsigned long long act = __ballot(pending != 0);
-            if (act == 0) break;
-            const int leader = __ffsll((long long)act) - 1;
-            const int first = (pending & 1) ? s[0] : (pending & 2) ? s[1] : s[2];
-            const int sec = __shfl(first, leader);
-            int hit = 0;
-            for (int j = 0; j < 3; ++j)
-                if ((pending >> j & 1) && s[j] == sec) hit = 1 << j;
-            const unsigned long long m = __ballot(hit != 0);
+        if (r < g.n) row_sectors(g, v[c], s, bad);
+        // Lanes sharing a sector take consecutive slots in lane order.
+        for_each_sector(s, [&](int sec, unsigned long long m, bool mine) {
             const uint32_t base = my[sec];
-            if (hit) {
+            if (mine) {
                 const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
                     (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
                 const uint32_t pos = base + rank;
                 if (pos < cap) rows[pos] = (uint32_t)r;
-                pending &= ~hit;
             }
             if (lane == 0) my[sec] = base + (uint32_t)__popcll(m);
-        }
+        });
     }
 }
 
@@ -422,6 +464,23 @@ k_gather_vis(const uint32_t *__restrict__ list, uint32_t nsec, uint32_t n, uint3
     const uint32_t src = list[nsec - 1 - ic];
     if (src >= n) return;
     const size_t from = (size_t)src * FP + k;
+    vrs[e] = vr[from];
+    vis[e] = vi[from];
+}
+
+// Same, four floats per thread when F*P is a multiple of 4 (16-byte aligned
+// row segments).
+__global__ void __launch_bounds__(kPrepThreads)
+k_gather_vis4(const uint32_t *__restrict__ list, uint32_t nsec, uint32_t n, uint32_t FP4,
+              const float4 *__restrict__ vr, const float4 *__restrict__ vi,
+              float4 *__restrict__ vrs, float4 *__restrict__ vis)
+{
+    const size_t e = (size_t)blockIdx.x * kPrepThreads + threadIdx.x;
+    if (e >= (size_t)nsec * FP4) return;
+    const uint32_t ic = (uint32_t)(e / FP4), k = (uint32_t)(e - (size_t)ic * FP4);
+    const uint32_t src = list[nsec - 1 - ic];
+    if (src >= n) return;
+    const size_t from = (size_t)src * FP4 + k;
     vrs[e] = vr[from];
     vis[e] = vi[from];
 }
@@ -619,10 +678,20 @@ int rick_sector_gather(const unsigned int *list, unsigned int nsec, int isector,
                        uu, vv, ww, weight, uus, vvs, wws, weightss);
     PREP_HIP(hipGetLastError());
     const uint32_t FP = freq_per_chan * polarizations;
-    const size_t ne = (size_t)nsec * FP;
-    hipLaunchKernelGGL(k_gather_vis, dim3((unsigned)((ne + kPrepThreads - 1) / kPrepThreads)),
-                       dim3(kPrepThreads), 0, s, list, nsec, num_points, FP, vis_real, vis_img,
-                       visreals, visimgs);
+    const bool vec = FP % 4 == 0 &&
+                     ((uintptr_t)vis_real | (uintptr_t)vis_img | (uintptr_t)visreals |
+                      (uintptr_t)visimgs) % 16 == 0;
+    const size_t ne = (size_t)nsec * (vec ? FP / 4 : FP);
+    const dim3 grd((unsigned)((ne + kPrepThreads - 1) / kPrepThreads));
+    if (vec)
+        hipLaunchKernelGGL(k_gather_vis4, grd, dim3(kPrepThreads), 0, s, list, nsec, num_points,
+                           FP / 4, reinterpret_cast<const float4 *>(vis_real),
+                           reinterpret_cast<const float4 *>(vis_img),
+                           reinterpret_cast<float4 *>(visreals),
+                           reinterpret_cast<float4 *>(visimgs));
+    else
+        hipLaunchKernelGGL(k_gather_vis, grd, dim3(kPrepThreads), 0, s, list, nsec, num_points,
+                           FP, vis_real, vis_img, visreals, visimgs);
     PREP_HIP(hipGetLastError());
     return RICK_OK;
 }

@@ -112,9 +112,9 @@ def test_sector_bin_empty_and_tiny():
     check_bins(np.array([0.5]), 1, 512, 1 / 512, 7)
 
 
-@pytest.mark.parametrize("isector", [0, 2])
-def test_sector_gather_matches_oracle(isector):
-    n, F, P, S = 70_001, 4, 2, 4
+@pytest.mark.parametrize("isector,F,P", [(0, 4, 2), (2, 4, 2), (1, 3, 1)])
+def test_sector_gather_matches_oracle(isector, F, P):
+    n, S = 70_001, 4
     rng = np.random.default_rng(isector)
     a = {"uu": rng.uniform(0, 1, n), "vv": rng.uniform(0, 1, n), "ww": rng.uniform(0, 1, n),
          "weight": rng.uniform(0.5, 2, n * P).astype(np.float32),
