@@ -566,12 +566,26 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, const uint32_t *__restrict__
         z.x = T(0);
         z.y = T(0);
         T2 *g2 = reinterpret_cast<T2 *>(grid);
+        // FP32 cells are 8 bytes: clear two per 16-byte store where the grid
+        // allows (16-byte aligned base, even row length and tile width).
+        const bool pairs = PREC == RICK_FP32 && (g.nx & 1) == 0 &&
+                           (reinterpret_cast<uintptr_t>(grid) & 15u) == 0u;
         for (uint32_t e = 0; e < r0; ++e) {
             const uint32_t be = b + e;
             const int we = (int)(be / per_plane), re = (int)(be % per_plane);
             const int je = (re % g.tiles_u) * TU, ke = (re / g.tiles_u) * TV;
             const int nje = min(TU, g.nx - je), nke = min(TV, g.ny - ke);
             T2 *t0 = g2 + ((size_t)we * g.ny + (size_t)ke) * (size_t)g.nx + (size_t)je;
+            if (pairs && (nje & 1) == 0) {
+                float4 *t4 = reinterpret_cast<float4 *>(t0);
+                const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                constexpr int HU = TU / 2;
+                for (int idx = threadIdx.x; idx < nke * HU; idx += kTileThreads) {
+                    const int kk = idx / HU, jj = idx % HU;
+                    if (2 * jj < nje) t4[(size_t)kk * (size_t)(g.nx / 2) + jj] = z4;
+                }
+                continue;
+            }
             for (int idx = threadIdx.x; idx < nke * TU; idx += kTileThreads) {
                 const int kk = idx / TU, jj = idx % TU;
                 if (jj < nje) t0[(size_t)kk * g.nx + jj] = z;
