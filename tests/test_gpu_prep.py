@@ -132,6 +132,40 @@ def test_sector_gather_matches_oracle(isector, F, P):
         np.testing.assert_array_equal(host(got[k]), want[k])
 
 
+@pytest.mark.parametrize("precision", [FP64, FP32])
+def test_sector_slabs_grid_as_ranks_do(precision):
+    # Four v-slabs of a 256^2 grid, as four ranks would grid them: each
+    # slab's rows (halo included), walked backwards and shifted into the slab
+    # (gridding_cpu.c:118-124), gridded into a 256 x 64 slab whose halo rows
+    # reach below v = 0.  FP64 within 1e-12 of the oracle, FP32 within 1e-5.
+    from rick_amd import Plan
+    cfg = synth.CONFIGS["small"].scaled(n_ant=24, n_times=64, grid=256, planes=4,
+                                        kernel=KAISERBESSEL, n_chan=2, n_pol=1)
+    ds = synth.make_dataset(cfg)
+    S, nx, F, P = 4, 256, cfg.n_chan, cfg.n_pol
+    yaxis, dx, dw = nx // S, 1.0 / nx, 1.0 / cfg.planes
+    keys = ("uu", "vv", "ww", "weight", "vis_real", "vis_img")
+    h = {k: np.ascontiguousarray(getattr(ds, k)) for k in keys}
+    d = {k: dev(v) for k, v in h.items()}
+    b = prep.sector_bin(d["vv"], S, yaxis, dx, cfg.support)
+    histo, start, rows, _ = O.sector_bin(h["vv"], S, yaxis, dx, cfg.support)
+    assert histo[:S].sum() > h["uu"].size                  # halo rows present
+    for s in range(S):
+        print(f"case slab {s} precision={precision}", flush=True)
+        g = prep.sector_gather(b.sector(s), s, yaxis, dx, F, P, *(d[k] for k in keys))
+        o = O.sector_gather(rows[start[s]:start[s] + histo[s]], s, yaxis, dx, F, P,
+                            *(h[k] for k in keys))
+        assert host(g["vv"]).min() < 0.0 or s == 0
+        plan = Plan(nx, yaxis, cfg.planes, kernel=KAISERBESSEL, support=cfg.support, dx=dx,
+                    dw=dw, precision=precision, max_points=int(histo[s]))
+        grid = plan.grid(g["uu"], g["vv"], g["ww"], g["vis_real"], g["vis_img"], g["weight"],
+                         F, P)
+        want = O.wstack(KAISERBESSEL, cfg.planes, o["uu"], o["vv"], o["ww"], o["vis_real"],
+                        o["vis_img"], o["weight"], F, P, dx, dw, cfg.support, nx, yaxis)
+        err = rel_l2(host(grid), want)
+        assert err <= (1e-12 if precision == FP64 else 1e-5), err
+
+
 # --- binMS reads into device memory, cube writes from device grids ------------
 
 def test_binms_device_read_matches_host_read(tmp_path):
