@@ -8,11 +8,11 @@
 //
 //   k_count      rows -> buckets (bucket = w-plane x TUxTV grid tile whose core
 //                the row's footprint touches; at most 4 per row).  Each
-//                workgroup aggregates the keys of its 8192 rows in an LDS hash
+//                workgroup aggregates the keys of its 16384 rows in an LDS hash
 //                table and adds one count per distinct key to the global
 //                histogram: device-scope atomics are slow on this chip (its
-//                eight L2 slices are not coherent with each other), and 8192
-//                consecutive rows touch only a few hundred distinct buckets.
+//                eight L2 slices are not coherent with each other), and 16384
+//                consecutive rows touch a few thousand distinct buckets.
 //   k_scan_*     multi-block exclusive scans of the histogram -> bucket
 //                offsets, the task list (a task is <= rmax records of one
 //                bucket, or a zero task), the list of split buckets.
@@ -54,11 +54,12 @@ constexpr int kRowsPerBlock = kScatterThreads * kRowsPerThread;
 constexpr int kHashBits = 12;
 constexpr int kHashSlots = 1 << kHashBits;
 constexpr uint32_t kDirect = 0x80000000u;     // item flag: key without a hash slot
-// Count: 8192-row blocks, bounded probing, overflow -> direct global atomic.
-constexpr int kBinThreads = 512;
+// Count: 16384-row blocks, 8192-slot hash with bounded probing, overflow ->
+// direct global atomic.
+constexpr int kBinThreads = 1024;
 constexpr int kBinRowsPerThread = 16;
 constexpr int kBinRows = kBinThreads * kBinRowsPerThread;
-constexpr int kBinHashBits = 11;
+constexpr int kBinHashBits = 13;
 constexpr int kBinSlots = 1 << kBinHashBits;
 constexpr int kMaxProbe = 32;
 constexpr uint32_t kZeroTask = 2u;            // task.w of a zero task
@@ -148,6 +149,24 @@ __device__ __forceinline__ uint32_t hinsert_bounded(uint32_t *hkey, uint32_t key
     return kEmpty;
 }
 
+// Same, reading the slot before trying to claim it: a key already present
+// costs an LDS read instead of a compare-and-swap.
+template <int HB>
+__device__ __forceinline__ uint32_t hinsert_peek(uint32_t *hkey, uint32_t key)
+{
+    uint32_t h = (key * 2654435761u) >> (32 - HB);
+    for (int p = 0; p < kMaxProbe; ++p) {
+        const uint32_t cur = hkey[h];
+        if (cur == key) return h;
+        if (cur == kEmpty) {
+            const uint32_t old = atomicCAS(&hkey[h], kEmpty, key);
+            if (old == kEmpty || old == key) return h;
+        }
+        h = (h + 1) & ((1u << HB) - 1u);
+    }
+    return kEmpty;
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -172,7 +191,7 @@ k_count(GridGeom g, uint32_t n, const double *__restrict__ uu, const double *__r
             uint32_t keys[4];
             const int nk = row_keys(g, f, keys);
             for (int q = 0; q < nk; ++q) {
-                const uint32_t h = hinsert_bounded<kBinHashBits>(hkey, keys[q]);
+                const uint32_t h = hinsert_peek<kBinHashBits>(hkey, keys[q]);
                 if (h == kEmpty) atomicAdd(&counts[keys[q]], 1u);
                 else atomicAdd(&hcnt[h], 1u);
             }
