@@ -244,6 +244,25 @@ def prep_rates(ds, cfg, torch, steps=5):
     return out
 
 
+def copy_bandwidth(torch, gib=1, reps=5):
+    """Device-to-device copy of ``gib`` GiB (read + write counted), timed with
+    events: the practical HBM ceiling beside the published 8 TB/s."""
+    n = (gib << 30) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    b.synchronize()
+    ms = a.elapsed_time(b) / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * (gib << 30) / ms / 1e6
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -304,6 +323,8 @@ def main():
         del plan
         torch.cuda.empty_cache()
 
+    copy_gbs = copy_bandwidth(torch)
+    log(f"device copy: {copy_gbs:.0f} GB/s (read + write)")
     main_path = "fp32" if "fp32" in res else "fp64"
     r = res[main_path]
     rows, F, P = cfg.n_rows, cfg.n_chan, cfg.n_pol
@@ -342,6 +363,8 @@ def main():
                    "step": "one wstack() gridding pass over all visibilities",
                    "path": main_path},
         "roofline": {"bound": "hbm", "kernel": dom,
+                     "measured_copy_GBs": copy_gbs,
+                     "frac_of_measured_copy": (achieved / copy_gbs) if achieved else None,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": measured_traffic(dom, cfg.name, main_path),

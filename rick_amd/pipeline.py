@@ -36,7 +36,11 @@ class Result:
 def image_from_binms(path, grid_size: int, planes: int, kernel=KAISERBESSEL, support: int = 7,
                      precision: int = FP64, drop: int = binms.DROP_ROWS, normalize: bool = True,
                      phase_on: bool = True, device: int = 0, out_dir=None,
-                     write_data: bool = False) -> Result:
+                     write_data: bool = False, nsectors: int = 1) -> Result:
+    """``nsectors`` > 1 grids the rows as that many ranks would: v-slabs of
+    grid_size/nsectors rows, each with its halo rows (gridding_cpu.c:90-204),
+    gridded one after the other on this GPU into their place in the grid (the
+    reduce onto the slab's owner becomes a placement)."""
     import torch
     t = {}
 
@@ -54,21 +58,36 @@ def image_from_binms(path, grid_size: int, planes: int, kernel=KAISERBESSEL, sup
     dx, dw = 1.0 / nx, 1.0 / planes                      # init.c:11-12
     if normalize:
         prep.normalize_uvw(d["uu"], d["vv"], d["ww"])
-    # One rank: one sector, the whole grid (yaxis = Ny); the reference still
-    # routes the rows through the binning and walks them backwards.
-    bins = prep.sector_bin(d["vv"], 1, ny, dx, support)
-    rows = prep.sector_gather(bins.sector(0), 0, ny, dx, meta.freq_per_chan, meta.polarisations,
-                              d["uu"], d["vv"], d["ww"], d["weight"], d["vis_real"],
-                              d["vis_img"])
+    if nsectors < 1 or ny % nsectors:
+        raise ValueError(f"{nsectors} sectors do not divide {ny} grid rows")
+    yaxis = ny // nsectors
+    # With one sector the whole grid is one slab (yaxis = Ny); the reference
+    # still routes the rows through the binning and walks them backwards.
+    bins = prep.sector_bin(d["vv"], nsectors, yaxis, dx, support)
+    sectors = [prep.sector_gather(bins.sector(s), s, yaxis, dx, meta.freq_per_chan,
+                                  meta.polarisations, d["uu"], d["vv"], d["ww"], d["weight"],
+                                  d["vis_real"], d["vis_img"]) for s in range(nsectors)]
     del d
     t0 = mark("prepare", t0)
     kind = KERNELS.get(kernel, kernel) if isinstance(kernel, str) else kernel
-    n = int(rows["uu"].numel())
-    plan = Plan(nx, ny, planes, kernel=kind, support=support, dx=dx, dw=dw, precision=precision,
-                max_points=max(n, 1), resolution=meta.resolution, wmin=meta.wmin,
-                wmax=meta.wmax, phase_on=phase_on, device=device)
-    grid = plan.grid(rows["uu"], rows["vv"], rows["ww"], rows["vis_real"], rows["vis_img"],
-                     rows["weight"], meta.freq_per_chan, meta.polarisations)
+    n = int(sum(int(r["uu"].numel()) for r in sectors))   # halo rows count once per slab
+    common = dict(kernel=kind, support=support, dx=dx, dw=dw, precision=precision,
+                  resolution=meta.resolution, wmin=meta.wmin, wmax=meta.wmax,
+                  phase_on=phase_on, device=device)
+    plan = Plan(nx, ny, planes, max_points=1 if nsectors > 1 else max(n, 1), **common)
+    F, P = meta.freq_per_chan, meta.polarisations
+    if nsectors == 1:
+        r = sectors[0]
+        grid = plan.grid(r["uu"], r["vv"], r["ww"], r["vis_real"], r["vis_img"], r["weight"], F, P)
+    else:
+        grid = plan.new_grid()
+        g3 = grid.view(planes, ny, 2 * nx)
+        cap = max(1, max(int(r["uu"].numel()) for r in sectors))
+        slab_plan = Plan(nx, yaxis, planes, max_points=cap, **common)
+        for s, r in enumerate(sectors):
+            slab = slab_plan.grid(r["uu"], r["vv"], r["ww"], r["vis_real"], r["vis_img"],
+                                  r["weight"], F, P)
+            g3[:, s * yaxis:(s + 1) * yaxis, :] = slab.view(planes, yaxis, 2 * nx)
     t0 = mark("grid", t0)
     if out_dir is not None and write_data:
         os.makedirs(out_dir, exist_ok=True)
@@ -88,7 +107,8 @@ def image_from_binms(path, grid_size: int, planes: int, kernel=KAISERBESSEL, sup
         binms.write_image(os.path.join(out_dir, "fft_real.bin"),
                           os.path.join(out_dir, "fft_img.bin"), re, im, nx, ny, 0)
         t0 = mark("write", t0)
-    return Result(re, im, grid, meta, n, t)
+    t["entries"] = n
+    return Result(re, im, grid, meta, nrows, t)
 
 
 def main():
@@ -103,10 +123,13 @@ def main():
     ap.add_argument("--no-normalize", action="store_true")
     ap.add_argument("--write-data", action="store_true")
     ap.add_argument("--out", default=None)
+    ap.add_argument("--sectors", type=int, default=1,
+                    help="grid as this many v-slabs, as that many ranks would")
     a = ap.parse_args()
     r = image_from_binms(a.path, a.grid, a.planes, kernel=a.kernel, support=a.support,
                          precision=FP32 if a.precision == "fp32" else FP64, drop=a.drop,
-                         normalize=not a.no_normalize, out_dir=a.out, write_data=a.write_data)
+                         normalize=not a.no_normalize, out_dir=a.out, write_data=a.write_data,
+                         nsectors=a.sectors)
     print(json.dumps({"rows": r.rows, "timings_ms": r.timings_ms}))
 
 
