@@ -314,6 +314,11 @@ __device__ __forceinline__ void pass(C2<T> *vals, int tid, const Lines &ln, int 
 // Lines per workgroup: 4096 elements per workgroup (256 threads), or one
 // line of 8192 (512 threads).
 constexpr int lines_per_wg(int L) { return L >= 4096 ? 1 : 4096 / L; }
+// Columns per workgroup of a plain column pass: FP32 lines of 64 or 128
+// elements take 8192 elements (512 threads), so each row segment a
+// workgroup reads or writes is 1 KB, as in FP64.
+template <typename T>
+constexpr int col_lines(int L) { return sizeof(T) == 4 && L >= 64 && L <= 128 ? 8192 / L : lines_per_wg(L); }
 constexpr int lds_elems(int L, bool rows)
 {
     return rows ? lines_per_wg(L) * (lpad(L) + 1) : lines_per_wg(L) * L;
@@ -425,13 +430,12 @@ struct ColArgs {
     int ts;                        // table stride for the L-point DFT
 };
 
-template <typename T, int L>
-__global__ void __launch_bounds__(L >= 8192 ? 512 : 256)
+template <typename T, int L, int G>
+__global__ void __launch_bounds__(G * L / kE)
 k_fft_cols(ColArgs a, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gout, T scale,
            ColOcc co)
 {
-    constexpr int G = lines_per_wg(L);
-    __shared__ C2<T> lds[lds_elems(L, false)];
+    __shared__ C2<T> lds[G * L];
     __shared__ unsigned char rl[256];
     const int c0 = blockIdx.x * G;
     const int s = blockIdx.y;
@@ -623,8 +627,8 @@ hipError_t launch_cols(int L, const ColArgs &a, int sets, int planes, const C2<T
 {
 #define RICK_COLS(LL)                                                                        \
     case LL: {                                                                               \
-        constexpr int G = lines_per_wg(LL);                                                  \
-        hipLaunchKernelGGL((k_fft_cols<T, LL>), dim3((a.nx + G - 1) / G, sets, planes),      \
+        constexpr int G = col_lines<T>(LL);                                                  \
+        hipLaunchKernelGGL((k_fft_cols<T, LL, G>), dim3((a.nx + G - 1) / G, sets, planes),   \
                            dim3(G * LL / kE), 0, s, a, tw, in, out, T(1), co);               \
         return hipGetLastError();                                                            \
     }
