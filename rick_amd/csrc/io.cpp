@@ -181,10 +181,11 @@ int rick_binms_read(const char *dir, unsigned int freq_per_chan, unsigned int po
                     float *weight, float *vis_real, float *vis_img, int device)
 {
     clear_status();
-    if (!uu || !vv || !ww || !weight || !vis_real || !vis_img)
-        return set_status(RICK_ERR_INVALID, "null array");
     if (freq_per_chan == 0 || polarizations == 0)
         return set_status(RICK_ERR_INVALID, "freq_per_chan and polarizations must be >= 1");
+    if (nrows == 0) return RICK_OK;
+    if (!uu || !vv || !ww || !weight || !vis_real || !vis_img)
+        return set_status(RICK_ERR_INVALID, "null array");
     const size_t F = freq_per_chan, P = polarizations, r0 = startrow, n = nrows;
     // Byte offsets and sizes of init.c:429-488.
     void *dst[6] = {uu, vv, ww, weight, vis_real, vis_img};

@@ -116,6 +116,7 @@ def test_binms_short_file_is_an_error(tmp_path):
     assert "short read" in str(e.value)
     got = binms.read(tmp_path, m, 0, n - 1)          # rows that are there still read
     assert got["uu"].size == n - 1
+    assert binms.read(tmp_path, m, n, 0)["vis_img"].size == 0   # no rows: nothing read
 
 
 def test_write_image_offsets(tmp_path):
