@@ -248,9 +248,22 @@ __device__ __forceinline__ void load_first(C2<T> *vals, int tid, const Lines &ln
     }
 }
 
-// One pass of the transform and, recursively, the rest.  With PRE the first
-// pass's inputs are already in vals (load_first), so it issues no loads.
-template <typename T, int L, int G, bool ROWS, int P, typename Sink, bool PRE = false>
+// LDS slot of element n of line g between passes.
+template <bool ROWS, int L, int G>
+__device__ __forceinline__ int lds_at(int g, int n)
+{
+    if constexpr (ROWS) return g * (lpad(L) + 1) + lpad(n);
+    else return n * G + g;
+}
+
+// One pass of the transform and, recursively, the rest.  With PRE the pass's
+// inputs are already in vals (preloaded by load_first, or placed by a split
+// exchange), so it issues no loads.  With SPLIT the exchange between passes
+// moves the real parts through LDS, then the imaginary parts, so the LDS
+// buffer holds one T per element (half the bytes) at the cost of two more
+// barriers per exchange.
+template <typename T, int L, int G, bool ROWS, int P, typename Sink, bool PRE = false,
+          bool SPLIT = false>
 __device__ __forceinline__ void pass(C2<T> *vals, int tid, const Lines &ln, int ts,
                                      const C2<T> *__restrict__ tw, C2<T> *lds, const C2<T> *gin,
                                      Sink &sink)
@@ -259,34 +272,32 @@ __device__ __forceinline__ void pass(C2<T> *vals, int tid, const Lines &ln, int 
     if constexpr (P < NP) {
         constexpr int R = radix_of(L, P), Ns = ns_of(L, P), LR = L / R, BPT = kE / R;
         constexpr int NT = G * L / kE;
-        constexpr int LP = lpad(L) + 1;
         constexpr bool from_lds = P > 0, to_lds = P + 1 < NP;
-        if constexpr (!from_lds && PRE) {
-            // inputs preloaded
-        } else {
+        if constexpr (!PRE) {
 #pragma unroll
-        for (int i = 0; i < BPT; ++i) {
-            int g, j;
-            bmap<ROWS, LR, G>(tid + i * NT, g, j);
-            C2<T> *v = vals + i * R;
+            for (int i = 0; i < BPT; ++i) {
+                int g, j;
+                bmap<ROWS, LR, G>(tid + i * NT, g, j);
+                C2<T> *v = vals + i * R;
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int n = j + r * LR;
-                if constexpr (from_lds) {
-                    v[r] = ROWS ? lds[g * LP + lpad(n)] : lds[n * G + g];
-                } else {
-                    const bool live =
-                        g < ln.nvalid &&
-                        (ln.occ == nullptr || ln.occ[g * ln.occ_stride + n / ln.occ_tu]) &&
-                        (ln.rowlive == nullptr ||
-                         ln.rowlive[(ln.rl_row0 + n * ln.rl_rstride) >> ln.rl_shift]);
-                    v[r] = live ? gin[ln.in_base + (size_t)g * ln.in_line + (size_t)n * ln.in_el]
-                                : C2<T>{T(0), T(0)};
+                for (int r = 0; r < R; ++r) {
+                    const int n = j + r * LR;
+                    if constexpr (from_lds) {
+                        v[r] = lds[lds_at<ROWS, L, G>(g, n)];
+                    } else {
+                        const bool live =
+                            g < ln.nvalid &&
+                            (ln.occ == nullptr || ln.occ[g * ln.occ_stride + n / ln.occ_tu]) &&
+                            (ln.rowlive == nullptr ||
+                             ln.rowlive[(ln.rl_row0 + n * ln.rl_rstride) >> ln.rl_shift]);
+                        v[r] = live ? gin[ln.in_base + (size_t)g * ln.in_line +
+                                          (size_t)n * ln.in_el]
+                                    : C2<T>{T(0), T(0)};
+                    }
                 }
             }
+            if constexpr (from_lds) __syncthreads();
         }
-        }
-        if constexpr (from_lds) __syncthreads();
 #pragma unroll
         for (int i = 0; i < BPT; ++i) {
             int g, j;
@@ -294,20 +305,53 @@ __device__ __forceinline__ void pass(C2<T> *vals, int tid, const Lines &ln, int 
             C2<T> *v = vals + i * R;
             if constexpr (Ns > 1) twiddle<T, R>(v, j % Ns, (L / (Ns * R)) * ts, tw);
             dftR<T, R>(v);
-            const int dbase = (j / Ns) * Ns * R + j % Ns;
+            if constexpr (!(to_lds && SPLIT)) {
+                const int dbase = (j / Ns) * Ns * R + j % Ns;
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int n = dbase + r * Ns;
-                if constexpr (to_lds) {
-                    if (ROWS) lds[g * LP + lpad(n)] = v[r];
-                    else lds[n * G + g] = v[r];
-                } else {
-                    sink(g, n, i * R + r, v[r]);
+                for (int r = 0; r < R; ++r) {
+                    const int n = dbase + r * Ns;
+                    if constexpr (to_lds) lds[lds_at<ROWS, L, G>(g, n)] = v[r];
+                    else sink(g, n, i * R + r, v[r]);
                 }
             }
         }
-        if constexpr (to_lds) __syncthreads();
-        pass<T, L, G, ROWS, P + 1>(vals, tid, ln, ts, tw, lds, gin, sink);
+        if constexpr (to_lds && SPLIT) {
+            // Outputs of this pass are in vals; move them to the next pass's
+            // layout one component at a time.
+            constexpr int R2 = radix_of(L, P + 1), LR2 = L / R2, BPT2 = kE / R2;
+            T *lt = reinterpret_cast<T *>(lds);
+#pragma unroll
+            for (int comp = 0; comp < 2; ++comp) {
+#pragma unroll
+                for (int i = 0; i < BPT; ++i) {
+                    int g, j;
+                    bmap<ROWS, LR, G>(tid + i * NT, g, j);
+                    const int dbase = (j / Ns) * Ns * R + j % Ns;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const C2<T> o = vals[i * R + r];
+                        lt[lds_at<ROWS, L, G>(g, dbase + r * Ns)] = comp == 0 ? o.x : o.y;
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int i = 0; i < BPT2; ++i) {
+                    int g, j;
+                    bmap<ROWS, LR2, G>(tid + i * NT, g, j);
+#pragma unroll
+                    for (int r = 0; r < R2; ++r) {
+                        const T x = lt[lds_at<ROWS, L, G>(g, j + r * LR2)];
+                        if (comp == 0) vals[i * R2 + r].x = x;
+                        else vals[i * R2 + r].y = x;
+                    }
+                }
+                __syncthreads();
+            }
+            pass<T, L, G, ROWS, P + 1, Sink, true, true>(vals, tid, ln, ts, tw, lds, gin, sink);
+        } else {
+            if constexpr (to_lds) __syncthreads();
+            pass<T, L, G, ROWS, P + 1, Sink, false, SPLIT>(vals, tid, ln, ts, tw, lds, gin, sink);
+        }
     }
 }
 
@@ -378,7 +422,12 @@ k_fft_rows(int rows_total, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T>
     constexpr int G = lines_per_wg(L);
     constexpr int NT = G * L / kE;
     constexpr int MAXT = 256;  // tiles per row held in LDS (tile width >= 32, L <= 8192)
-    __shared__ C2<T> lds[lds_elems(L, true)];
+    // FP64 exchanges split into real and imaginary halves: one T per element
+    // in LDS, so twice the workgroups fit per CU (1.61 -> 1.35 ms on 32
+    // planes of 4096^2).  FP32 keeps whole exchanges, which measured as fast
+    // on large grids and faster on small ones.
+    constexpr bool kSplit = sizeof(T) == 8;
+    __shared__ C2<T> lds[kSplit ? (lds_elems(L, true) + 1) / 2 : lds_elems(L, true)];
     __shared__ unsigned char occ[oc_bytes(L)];
     Lines ln;
     const size_t row0 = (size_t)blockIdx.x * G;
@@ -418,7 +467,8 @@ k_fft_rows(int rows_total, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T>
     }
     C2<T> vals[kE];
     GlobalSink<T> sink{gout, &ln, tw, scale};
-    pass<T, L, G, true, 0>(vals, threadIdx.x, ln, 1, tw, lds, gin, sink);
+    pass<T, L, G, true, 0, GlobalSink<T>, false, kSplit>(vals, threadIdx.x, ln, 1, tw, lds, gin,
+                                                           sink);
 }
 
 // Columns: workgroup (x: strip of G columns, y: row set s, z: plane).
@@ -435,7 +485,8 @@ __global__ void __launch_bounds__(G * L / kE)
 k_fft_cols(ColArgs a, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gout, T scale,
            ColOcc co)
 {
-    __shared__ C2<T> lds[G * L];
+    constexpr bool kSplit = sizeof(T) == 8;   // as in k_fft_rows
+    __shared__ C2<T> lds[kSplit ? G * L / 2 : G * L];
     __shared__ unsigned char rl[256];
     const int c0 = blockIdx.x * G;
     const int s = blockIdx.y;
@@ -452,7 +503,8 @@ k_fft_cols(ColArgs a, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gou
     col_rowlive(co, blockIdx.z, s * a.in_set_mul, a.in_stride, rl, ln);
     C2<T> vals[kE];
     GlobalSink<T> sink{gout, &ln, tw, scale};
-    pass<T, L, G, false, 0>(vals, threadIdx.x, ln, a.ts, tw, lds, gin, sink);
+    pass<T, L, G, false, 0, GlobalSink<T>, false, kSplit>(vals, threadIdx.x, ln, a.ts, tw, lds,
+                                                            gin, sink);
 }
 
 // Last column pass fused with the phase correction and plane sum
