@@ -52,6 +52,8 @@ def parse():
                     help="autotune tile shape and rows/task first (rick_amd.autotune)")
     ap.add_argument("--no-prep", action="store_true",
                     help="skip the device preparation rates (normalize, binning, gather)")
+    ap.add_argument("--binms", default=None, metavar="DIR",
+                    help="also time a binMS write and read-back of the workload under DIR")
     ap.add_argument("--no-dropin", action="store_true",
                     help="skip the host-pointer wstack() drop-in timing (PCIe-inclusive)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -244,6 +246,35 @@ def prep_rates(ds, cfg, torch, steps=5):
     return out
 
 
+def binms_rates(ds, cfg, torch, path):
+    """binMS round trip on the workload's rows (physical values do not matter
+    for the rate): write the six files, read them back into host memory and
+    into device memory.  Rates count the bytes of the six files; the page
+    cache will usually hold them, so this is memory and PCIe, not disk."""
+    import shutil
+
+    from rick_amd import binms
+    h = ds.to_numpy()
+    F, P = cfg.n_chan, cfg.n_pol
+    t0 = time.perf_counter()
+    meta = binms.write(path, h.uu, h.vv, h.ww, h.weight, h.vis_real, h.vis_img, F, P)
+    tw = time.perf_counter() - t0
+    nbytes = sum(os.path.getsize(os.path.join(path, f)) for f in binms.FILES)
+    t0 = time.perf_counter()
+    binms.read(path, meta, 0, meta.nmeasures)
+    th = time.perf_counter() - t0
+    binms.read(path, meta, 0, meta.nmeasures, device=0)   # warm-up: pinned buffers
+    t0 = time.perf_counter()
+    d = binms.read(path, meta, 0, meta.nmeasures, device=0)
+    torch.cuda.synchronize()
+    td = time.perf_counter() - t0
+    ok = bool(torch.equal(d["vis_img"], ds.vis_img))
+    del d, h
+    shutil.rmtree(path, ignore_errors=True)
+    return {"bytes": nbytes, "write_GBs": nbytes / tw / 1e9, "read_host_GBs": nbytes / th / 1e9,
+            "read_device_GBs": nbytes / td / 1e9, "device_read_matches": ok}
+
+
 def copy_bandwidth(torch, gib=1, reps=5):
     """Device-to-device copy of ``gib`` GiB (read + write counted), timed with
     events: the practical HBM ceiling beside the published 8 TB/s."""
@@ -394,6 +425,9 @@ def main():
     if not args.no_prep:
         log("device preparation: normalize, sector binning, gather")
         out["prep"] = prep_rates(ds, cfg, torch)
+    if args.binms:
+        log(f"binMS write and read-back under {args.binms}")
+        out["binms"] = binms_rates(ds, cfg, torch, args.binms)
     if not args.no_dropin and not args.only:
         log("drop-in wstack() on host arrays")
         out["dropin_pcie_inclusive"] = dropin_rate(ds, cfg, torch)
