@@ -1,0 +1,137 @@
+"""Full-size checks on the LOFAR-like workload (7,745,536 rows x 16 channels =
+123,928,576 visibilities onto 4096^2 x 32 w-planes, Kaiser-Bessel support 7),
+where the CPU oracle would take minutes: properties that hold at any size.
+
+* Conservation through binning and tiling.  Every cell gets S*Ku[j]*Kv[k]
+  from each row whose footprint covers it (w-stacking.cu:499-532).  So the
+  sum of a w-plane's cells is sum_rows S * (sum_j Ku[j]) * (sum_k Kv[k]) over
+  the rows of that plane.  The marginal sums follow each row's clipped
+  footprint and table indices (w-stacking.cu:464-493); they are formed here
+  with torch, for all rows at once.  A lost, duplicated or misplaced record
+  would break the sums.
+* FFT: Parseval (sum |X|^2 = sum |x|^2 / (Nx*Ny) for the 1/(Nx*Ny)-scaled
+  transform) and the DC term (X[0,0] = sum x / (Nx*Ny)), per plane.
+* The fused image chain against the separate grid -> FFT -> phase path.
+* The FP32 image against the FP64 image: relative L2 <= 1e-5.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from rick_amd import FP32, FP64, plan_for, synth
+
+pytestmark = pytest.mark.gpu
+
+KL_OVERSAMPLE = 5
+
+
+@pytest.fixture(scope="module")
+def lofar():
+    cfg = synth.CONFIGS["lofar"]
+    print(f"case full-size {cfg.name}: {cfg.n_rows} rows, {cfg.n_vis} vis", flush=True)
+    ds = synth.make_dataset(cfg, device="cuda:0")
+    torch.cuda.synchronize()
+    yield ds
+    del ds
+    torch.cuda.empty_cache()
+
+
+def marginal_sums(pos, n, kl, table):
+    """Per row: sum of the table weights over its clipped footprint on one
+    axis (positions in cells, truncation toward zero)."""
+    g = torch.trunc(pos)
+    lo = torch.where(g > kl - 1, g - kl, torch.zeros_like(g))
+    hi = torch.where(g < n - kl, g + kl, torch.full_like(g, n - 1))
+    total = torch.zeros_like(pos)
+    for d in range(2 * kl + 1):
+        c = g - kl + d
+        dist = c + 0.5 - pos
+        idx = torch.trunc(KL_OVERSAMPLE * torch.abs(dist + kl))
+        ok = (c >= lo) & (c <= hi) & (idx >= 0) & (idx < table.numel())
+        w = table[idx.clamp(0, table.numel() - 1).long()]
+        total += torch.where(ok, w, torch.zeros_like(w))
+    return total
+
+
+def expected_plane_sums(ds):
+    cfg = ds.cfg
+    nx, planes, kl = cfg.grid, cfg.planes, (cfg.support - 1) // 2
+    table = torch.from_numpy(O.kernel_table(cfg.kernel, cfg.support)).to("cuda")
+    pu, pv = ds.uu / ds.dx, ds.vv / ds.dx
+    plane = torch.trunc(ds.ww / ds.dw)
+    ok = (plane >= 0) & (plane < planes)
+    # Channel/polarization sum: float products, accumulated in double.
+    F, P = cfg.n_chan, cfg.n_pol
+    wt = ds.weight.view(-1, 1, P)
+    s_re = (wt * ds.vis_real.view(-1, F, P)).double().sum(dim=(1, 2))
+    s_im = (wt * ds.vis_img.view(-1, F, P)).double().sum(dim=(1, 2))
+    w = marginal_sums(pu, nx, kl, table) * marginal_sums(pv, nx, kl, table)
+    w = torch.where(ok, w, torch.zeros_like(w))
+    idx = torch.where(ok, plane, torch.zeros_like(plane)).long()
+    re = torch.zeros(planes, dtype=torch.float64, device="cuda").index_add_(0, idx, s_re * w)
+    im = torch.zeros(planes, dtype=torch.float64, device="cuda").index_add_(0, idx, s_im * w)
+    return re.cpu().numpy(), im.cpu().numpy()
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32])
+def test_fullsize_grid_conserves_every_row(lofar, precision):
+    print(f"case full-size grid conservation precision={precision}", flush=True)
+    cfg = lofar.cfg
+    plan = plan_for(lofar, precision=precision)
+    grid = plan.grid(lofar.uu, lofar.vv, lofar.ww, lofar.vis_real, lofar.vis_img, lofar.weight,
+                     cfg.n_chan, cfg.n_pol)
+    g = grid.view(cfg.planes, -1, 2)
+    got_re = g[..., 0].double().sum(dim=1).cpu().numpy()
+    got_im = g[..., 1].double().sum(dim=1).cpu().numpy()
+    want_re, want_im = expected_plane_sums(lofar)
+    scale = np.abs(np.concatenate([want_re, want_im])).max()
+    err = np.abs(np.concatenate([got_re - want_re, got_im - want_im])).max() / scale
+    print(f"plane sums: max deviation {err:.3e} of the largest", flush=True)
+    assert err <= (1e-10 if precision == FP64 else 1e-5)
+    del grid, plan
+    torch.cuda.empty_cache()
+
+
+def test_fullsize_fft_parseval_and_dc(lofar):
+    cfg = lofar.cfg
+    plan = plan_for(lofar, precision=FP64)
+    grid = plan.grid(lofar.uu, lofar.vv, lofar.ww, lofar.vis_real, lofar.vis_img, lofar.weight,
+                     cfg.n_chan, cfg.n_pol)
+    n = cfg.grid * cfg.grid
+    g = grid.view(cfg.planes, n, 2)
+    energy_in = (g * g).sum(dim=(1, 2))
+    sum_in = g.sum(dim=1)                                   # (planes, 2)
+    out = plan.fft(grid, out=torch.empty_like(grid))
+    o = out.view(cfg.planes, n, 2)
+    energy_out = (o * o).sum(dim=(1, 2))
+    rel = ((energy_out * n - energy_in).abs() / energy_in).max().item()
+    dc = ((o[:, 0, :] * n - sum_in).abs().max() / sum_in.abs().max()).item()
+    print(f"Parseval max rel {rel:.3e}, DC max rel {dc:.3e}", flush=True)
+    assert rel <= 1e-10 and dc <= 1e-10
+    del grid, out, plan
+    torch.cuda.empty_cache()
+
+
+def test_fullsize_chain_and_precisions(lofar):
+    cfg = lofar.cfg
+    ins = (lofar.uu, lofar.vv, lofar.ww, lofar.vis_real, lofar.vis_img, lofar.weight,
+           cfg.n_chan, cfg.n_pol)
+    plan = plan_for(lofar, precision=FP64)
+    re64, im64, _ = plan.image(*ins)
+    # Separate path: grid, FFT in place, phase.
+    grid = plan.grid(*ins)
+    plan.fft(grid)
+    re_s, im_s = plan.phase(grid)
+    a = torch.cat([re64, im64])
+    b = torch.cat([re_s, im_s])
+    chain = (torch.linalg.norm(a - b) / torch.linalg.norm(b)).item()
+    del grid, plan
+    torch.cuda.empty_cache()
+    plan32 = plan_for(lofar, precision=FP32)
+    re32, im32, _ = plan32.image(*ins)
+    c = torch.cat([re32, im32])
+    fp32 = (torch.linalg.norm(c - a) / torch.linalg.norm(a)).item()
+    print(f"fused vs separate FP64 {chain:.3e}; FP32 vs FP64 image {fp32:.3e}", flush=True)
+    assert chain <= 1e-12
+    assert fp32 <= 1e-5
