@@ -81,15 +81,39 @@ def test_fullsize_grid_conserves_every_row(lofar, precision):
     plan = plan_for(lofar, precision=precision)
     grid = plan.grid(lofar.uu, lofar.vv, lofar.ww, lofar.vis_real, lofar.vis_img, lofar.weight,
                      cfg.n_chan, cfg.n_pol)
-    g = grid.view(cfg.planes, -1, 2)
-    got_re = g[..., 0].double().sum(dim=1).cpu().numpy()
-    got_im = g[..., 1].double().sum(dim=1).cpu().numpy()
-    want_re, want_im = expected_plane_sums(lofar)
-    scale = np.abs(np.concatenate([want_re, want_im])).max()
-    err = np.abs(np.concatenate([got_re - want_re, got_im - want_im])).max() / scale
+    err = plane_sum_deviation(lofar, grid)
     print(f"plane sums: max deviation {err:.3e} of the largest", flush=True)
     assert err <= (1e-10 if precision == FP64 else 1e-5)
     del grid, plan
+    torch.cuda.empty_cache()
+
+
+def plane_sum_deviation(ds, grid):
+    cfg = ds.cfg
+    g = grid.view(cfg.planes, -1, 2)
+    got_re = g[..., 0].sum(dim=1, dtype=torch.float64).cpu().numpy()
+    got_im = g[..., 1].sum(dim=1, dtype=torch.float64).cpu().numpy()
+    want_re, want_im = expected_plane_sums(ds)
+    scale = np.abs(np.concatenate([want_re, want_im])).max()
+    return np.abs(np.concatenate([got_re - want_re, got_im - want_im])).max() / scale
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32])
+def test_fullsize_jvla_grid_past_32bit_indices(precision):
+    # 8192^2 x 64 planes: 2*planes*Nx*Ny = 8.6e9 grid values, past the 2^31
+    # the reference's int index reaches (68.7 GB in FP64).
+    cfg = synth.CONFIGS["jvla"]
+    print(f"case full-size {cfg.name} precision={precision}: {cfg.n_vis} vis", flush=True)
+    ds = synth.make_dataset(cfg, device="cuda:0")
+    plan = plan_for(ds, precision=precision)
+    grid = plan.grid(ds.uu, ds.vv, ds.ww, ds.vis_real, ds.vis_img, ds.weight, cfg.n_chan,
+                     cfg.n_pol)
+    err = plane_sum_deviation(ds, grid)
+    # Cells past 2^31 hold data: the last plane with visibilities is non-zero.
+    last = grid.view(cfg.planes, -1)[-1].abs().max().item()
+    print(f"plane sums: max deviation {err:.3e}; last plane peak {last:.3e}", flush=True)
+    assert err <= (1e-10 if precision == FP64 else 1e-5)
+    del grid, plan, ds
     torch.cuda.empty_cache()
 
 
