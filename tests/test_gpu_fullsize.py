@@ -159,3 +159,28 @@ def test_fullsize_chain_and_precisions(lofar):
     print(f"fused vs separate FP64 {chain:.3e}; FP32 vs FP64 image {fp32:.3e}", flush=True)
     assert chain <= 1e-12
     assert fp32 <= 1e-5
+
+
+def test_fullsize_dropin_accumulates_into_host_grid(lofar):
+    # The reference's host-pointer wstack() at full size: it adds into the
+    # caller's grid (8.6 GB here), so two calls give twice the device grid.
+    from rick_amd import imaging
+    cfg = lofar.cfg
+    h = lofar.to_numpy()
+    nx = cfg.grid
+    plan = plan_for(lofar, precision=FP64)
+    dev = plan.grid(lofar.uu, lofar.vv, lofar.ww, lofar.vis_real, lofar.vis_img, lofar.weight,
+                    cfg.n_chan, cfg.n_pol).cpu().numpy()
+    del plan
+    torch.cuda.empty_cache()
+    grid = np.zeros(2 * cfg.planes * nx * nx)
+    imaging.set_kernel(cfg.kernel)
+    args = (cfg.planes, cfg.n_rows, cfg.n_chan, cfg.n_pol, h.uu, h.vv, h.ww, h.vis_real,
+            h.vis_img, h.weight, 1.0 / nx, 1.0 / cfg.planes, cfg.support, nx, nx, grid)
+    imaging.wstack(*args, num_threads=16)
+    one = float(np.linalg.norm(grid - dev) / np.linalg.norm(dev))
+    imaging.wstack(*args, num_threads=16)
+    dev *= 2.0
+    two = float(np.linalg.norm(grid - dev) / np.linalg.norm(dev))
+    print(f"drop-in vs device grid {one:.3e}; after a second call vs twice {two:.3e}", flush=True)
+    assert one <= 1e-13 and two <= 1e-13
