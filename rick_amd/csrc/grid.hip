@@ -548,7 +548,7 @@ __device__ __forceinline__ unsigned long long to_fixed(double x, double scale)
 // any cell that matters to the image) and the tile is rounded to FP32 once
 // when written.  A non-finite bound (NaN/Inf data) falls back to FP64
 // atomics.  SMAX >= support bounds the unrolled footprint loops.
-template <int PREC, int TU, int TV, int SMAX>
+template <int PREC, int TU, int TV, int SMAX, bool HIPREC>
 __global__ void __launch_bounds__(kTileThreads)
 k_tile(GridGeom g, const uint4 *__restrict__ tasks, const uint32_t *__restrict__ meta,
        const typename RecOf<PREC>::type *__restrict__ rec, const double *__restrict__ table,
@@ -651,7 +651,9 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, const uint32_t *__restrict__
     __syncthreads();
 
     const int S = g.support, KL = g.kl;
-    const bool hiprec = g.kind == RICK_GAUSS_HI_PRECISION;
+    // Per-cell Gaussian (GAUSS_HI_PRECISION) or table product: a template
+    // parameter, so each instantiation carries only its own cell arithmetic.
+    constexpr bool hiprec = HIPREC;
     // gauss_kernel_norm constants (w-stacking.cu:254-256).
     const double std22 = 1.0 / (2.0 * 1.0 * 1.0);
     const double gnorm = std22 / kRefPi;
@@ -778,13 +780,17 @@ hipError_t launch_tile_s(const GridGeom &g, const GridWork &w, const double *tab
     using Rec = typename RecOf<PREC>::type;
     const Rec *rec = static_cast<const Rec *>(w.records);
     const dim3 grd(w.task_cap), blk(kTileThreads);
-#define RICK_TILE_CASE(A, B)                                                               \
-    if (g.tu == A && g.tv == B) {                                                          \
-        tm->begin(RICK_STAGE_TILE, s);                                                     \
-        hipLaunchKernelGGL((k_tile<PREC, A, B, SMAX>), grd, blk, 0, s, g, w.tasks, w.meta, \
-                           rec, table, grid, accumulate);                                  \
-        tm->end(RICK_STAGE_TILE, s);                                                       \
-        return hipGetLastError();                                                          \
+#define RICK_TILE_CASE(A, B)                                                                \
+    if (g.tu == A && g.tv == B) {                                                           \
+        tm->begin(RICK_STAGE_TILE, s);                                                      \
+        if (g.kind == RICK_GAUSS_HI_PRECISION)                                              \
+            hipLaunchKernelGGL((k_tile<PREC, A, B, SMAX, true>), grd, blk, 0, s, g, w.tasks,  \
+                               w.meta, rec, table, grid, accumulate);                       \
+        else                                                                                \
+            hipLaunchKernelGGL((k_tile<PREC, A, B, SMAX, false>), grd, blk, 0, s, g, w.tasks, \
+                               w.meta, rec, table, grid, accumulate);                       \
+        tm->end(RICK_STAGE_TILE, s);                                                        \
+        return hipGetLastError();                                                           \
     }
     RICK_TILE_CASE(64, 64)
     RICK_TILE_CASE(64, 32)
