@@ -384,7 +384,7 @@ k_scan_down(GridGeom g, const uint32_t *__restrict__ counts,
     }
 }
 
-template <int PREC>
+template <int PREC, bool VEC4>
 __global__ void __launch_bounds__(kScatterThreads)
 k_scatter(GridGeom g, uint32_t n, uint32_t F, uint32_t P, const double *__restrict__ uu,
           const double *__restrict__ vv, const double *__restrict__ ww,
@@ -399,8 +399,9 @@ k_scatter(GridGeom g, uint32_t n, uint32_t F, uint32_t P, const double *__restri
     }
     __syncthreads();
     const uint32_t base = blockIdx.x * kRowsPerBlock;
-    const bool vec4 = ((F * P) % 4u) == 0u && ((reinterpret_cast<uintptr_t>(vr) |
-                                                reinterpret_cast<uintptr_t>(vi)) & 15u) == 0u;
+    // 16-byte visibility reads (F*P a multiple of 4, aligned arrays): chosen
+    // on the host, so each instantiation carries one read path.
+    constexpr bool vec4 = VEC4;
     double pu[kRowsPerThread], pv[kRowsPerThread], sre[kRowsPerThread], sim[kRowsPerThread];
     uint32_t item[kRowsPerThread][4];
     int nk[kRowsPerThread];
@@ -664,80 +665,91 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, const uint32_t *__restrict__
     // (often one baseline at consecutive times, i.e. the same cells) go to
     // different waves, so the lanes of one atomic instruction rarely collide.
     const uint32_t roff = (threadIdx.x & 63u) * (kTileThreads / 64) + (threadIdx.x >> 6);
-    for (uint32_t c0 = r0; c0 < r1; c0 += kTileThreads) {
-        const uint32_t r = c0 + roff;
-        if (r >= r1) continue;
-        const auto rc = rec[r];
-        const double pu = rc.pu, pv = rc.pv;
-        const T s_re = rc.sre, s_im = rc.sim;
-        Foot f;
-        uv_range(g, pu, pv, f);
-        const int ja = max(f.jmin, jlo), jb = min(f.jmax, jhi);
-        const int ka = max(f.kmin, klo), kb = min(f.kmax, khi);
-        const int ju0 = f.gu - KL, kv0 = f.gv - KL;
-        // u-weights of the footprint columns (w-stacking.cu:489, 492).
-        T wu[SMAX];
-        double ud[SMAX];
-        uint32_t umask = 0;
-#pragma unroll
-        for (int dj = 0; dj < SMAX; ++dj) {
-            const int j = ju0 + dj;
-            wu[dj] = T(0);
-            ud[dj] = 0.0;
-            if (dj < S && j >= ja && j <= jb) {
-                const double u_dist = (double)j + 0.5 - pu;
-                ud[dj] = u_dist;
-                const int jKer = (int)(kOversample * (fabs(u_dist + (double)KL)));
-                if (hiprec) {
-                    umask |= 1u << dj;
-                } else if (jKer >= 0 && jKer < g.ntable) {
-                    wu[dj] = stab[jKer];
-                    umask |= 1u << dj;
-                }
-            }
-        }
-        if (umask == 0) continue;
-#pragma unroll
-        for (int dk = 0; dk < SMAX; ++dk) {
-            const int k = kv0 + dk;
-            if (dk >= S || k < ka || k > kb) continue;
-            // w-stacking.cu:484, 493
-            const double v_dist = (double)k + 0.5 - pv;
-            T wv = T(0);
-            if (!hiprec) {
-                const int kKer = (int)(kOversample * (fabs(v_dist + (double)KL)));
-                if (kKer < 0 || kKer >= g.ntable) continue;
-                wv = stab[kKer];
-            }
-            // LDS index of footprint column 0 on this row; only columns with
-            // their umask bit set (inside the tile) are ever accessed.
-            const int rowbase = (k - k0) * LDU + (ju0 - j0);
-#pragma unroll
+    // The FP32 fixed-point and the FP64-atomic accumulation are separate
+    // loops, so the hot one carries no code of the other.
+    auto accumulate_all = [&](auto fixed_tag) {
+        constexpr bool FIXED = decltype(fixed_tag)::value;
+        for (uint32_t c0 = r0; c0 < r1; c0 += kTileThreads) {
+            const uint32_t r = c0 + roff;
+            if (r >= r1) continue;
+            const auto rc = rec[r];
+            const double pu = rc.pu, pv = rc.pv;
+            const T s_re = rc.sre, s_im = rc.sim;
+            Foot f;
+            uv_range(g, pu, pv, f);
+            const int ja = max(f.jmin, jlo), jb = min(f.jmax, jhi);
+            const int ka = max(f.kmin, klo), kb = min(f.kmax, khi);
+            const int ju0 = f.gu - KL, kv0 = f.gv - KL;
+            // u-weights of the footprint columns (w-stacking.cu:489, 492).
+            T wu[SMAX];
+            double ud[SMAX];
+            uint32_t umask = 0;
+    #pragma unroll
             for (int dj = 0; dj < SMAX; ++dj) {
-                if (!((umask >> dj) & 1u)) continue;
-                // conv_weight = K[jKer] * K[kKer] (w-stacking.cu:499/505), or
-                // gauss_kernel_norm per cell (w-stacking.cu:496).
-                const T c = hiprec ? (T)(gnorm * exp(-((ud[dj] * ud[dj]) + (v_dist * v_dist)) *
-                                                     std22))
-                                   : wu[dj] * wv;
-                const T are = s_re * c, aim = s_im * c;
-                if (PREC == RICK_FP32 && fixed) {
-                    __hip_atomic_fetch_add(
-                        reinterpret_cast<unsigned long long *>(&sre[rowbase + dj]),
-                        to_fixed((double)are, scale), __ATOMIC_RELAXED,
-                        __HIP_MEMORY_SCOPE_WORKGROUP);
-                    __hip_atomic_fetch_add(
-                        reinterpret_cast<unsigned long long *>(&sim[rowbase + dj]),
-                        to_fixed((double)aim, scale), __ATOMIC_RELAXED,
-                        __HIP_MEMORY_SCOPE_WORKGROUP);
-                } else {
-                    __hip_atomic_fetch_add(&sre[rowbase + dj], (double)are, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-                    __hip_atomic_fetch_add(&sim[rowbase + dj], (double)aim, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                const int j = ju0 + dj;
+                wu[dj] = T(0);
+                ud[dj] = 0.0;
+                if (dj < S && j >= ja && j <= jb) {
+                    const double u_dist = (double)j + 0.5 - pu;
+                    ud[dj] = u_dist;
+                    const int jKer = (int)(kOversample * (fabs(u_dist + (double)KL)));
+                    if (hiprec) {
+                        umask |= 1u << dj;
+                    } else if (jKer >= 0 && jKer < g.ntable) {
+                        wu[dj] = stab[jKer];
+                        umask |= 1u << dj;
+                    }
+                }
+            }
+            if (umask == 0) continue;
+    #pragma unroll
+            for (int dk = 0; dk < SMAX; ++dk) {
+                const int k = kv0 + dk;
+                if (dk >= S || k < ka || k > kb) continue;
+                // w-stacking.cu:484, 493
+                const double v_dist = (double)k + 0.5 - pv;
+                T wv = T(0);
+                if (!hiprec) {
+                    const int kKer = (int)(kOversample * (fabs(v_dist + (double)KL)));
+                    if (kKer < 0 || kKer >= g.ntable) continue;
+                    wv = stab[kKer];
+                }
+                // LDS index of footprint column 0 on this row; only columns with
+                // their umask bit set (inside the tile) are ever accessed.
+                const int rowbase = (k - k0) * LDU + (ju0 - j0);
+    #pragma unroll
+                for (int dj = 0; dj < SMAX; ++dj) {
+                    if (!((umask >> dj) & 1u)) continue;
+                    // conv_weight = K[jKer] * K[kKer] (w-stacking.cu:499/505), or
+                    // gauss_kernel_norm per cell (w-stacking.cu:496).
+                    const T c = hiprec ? (T)(gnorm * exp(-((ud[dj] * ud[dj]) + (v_dist * v_dist)) *
+                                                         std22))
+                                       : wu[dj] * wv;
+                    const T are = s_re * c, aim = s_im * c;
+                    if constexpr (FIXED) {
+                        __hip_atomic_fetch_add(
+                            reinterpret_cast<unsigned long long *>(&sre[rowbase + dj]),
+                            to_fixed((double)are, scale), __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_WORKGROUP);
+                        __hip_atomic_fetch_add(
+                            reinterpret_cast<unsigned long long *>(&sim[rowbase + dj]),
+                            to_fixed((double)aim, scale), __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_WORKGROUP);
+                    } else {
+                        __hip_atomic_fetch_add(&sre[rowbase + dj], (double)are, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                        __hip_atomic_fetch_add(&sim[rowbase + dj], (double)aim, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
                 }
             }
         }
+    };
+    if constexpr (PREC == RICK_FP32) {
+        if (fixed) accumulate_all(std::true_type{});
+        else accumulate_all(std::false_type{});
+    } else {
+        accumulate_all(std::false_type{});
     }
     __syncthreads();
 
@@ -904,16 +916,21 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
     tm->end(RICK_STAGE_SCAN, s);
     tm->begin(RICK_STAGE_SCATTER, s);
     if (n > 0) {
-        if (precision == RICK_FP64)
-            hipLaunchKernelGGL(k_scatter<RICK_FP64>, dim3(nblk), dim3(kScatterThreads), 0, s, g, n,
-                               F, P,
-                               uu, vv, ww, vr, vi, wt, w.cursor,
-                               static_cast<RecF64 *>(w.records));
-        else
-            hipLaunchKernelGGL(k_scatter<RICK_FP32>, dim3(nblk), dim3(kScatterThreads), 0, s, g, n,
-                               F, P,
-                               uu, vv, ww, vr, vi, wt, w.cursor,
-                               static_cast<RecF32 *>(w.records));
+        const bool vec4 = ((F * P) % 4u) == 0u &&
+                          ((reinterpret_cast<uintptr_t>(vr) | reinterpret_cast<uintptr_t>(vi)) &
+                           15u) == 0u;
+#define RICK_SCATTER(PR, V4)                                                                  \
+    hipLaunchKernelGGL((k_scatter<PR, V4>), dim3(nblk), dim3(kScatterThreads), 0, s, g, n, F, P, \
+                       uu, vv, ww, vr, vi, wt, w.cursor,                                      \
+                       static_cast<typename RecOf<PR>::type *>(w.records))
+        if (precision == RICK_FP64) {
+            if (vec4) RICK_SCATTER(RICK_FP64, true);
+            else RICK_SCATTER(RICK_FP64, false);
+        } else {
+            if (vec4) RICK_SCATTER(RICK_FP32, true);
+            else RICK_SCATTER(RICK_FP32, false);
+        }
+#undef RICK_SCATTER
     }
     tm->end(RICK_STAGE_SCATTER, s);
     if (!accumulate) {
