@@ -290,3 +290,35 @@ def test_normalize_uvw_matches_numpy_restatement():
     uu, vv, ww = a
     assert uu.min() > 0 and uu.max() < 1 and vv.min() > 0 and vv.max() < 1
     assert ww.min() == 0.0 and ww.max() == 1.0
+
+
+def test_normalize_uvw_closed_forms():
+    # gridding.c:37-82 fixed by hand on small cases.  The scale is the
+    # largest |extremum| of u and v, with the maxima folded from 0.0 and the
+    # minima from 1e20, plus 0.1 %; u, v map to (x + maxg) / (2 maxg); w maps
+    # its own extremes to exactly 0 and 1.
+    # All-negative u and v: the maxima stay 0.0, so the scale comes from the
+    # minima: maxg = 4 * 1.001, and u = -4 maps to 0.004 / 8.008 = 1/2002.
+    u = np.array([-4.0, -2.0])
+    v = np.array([-1.0, -3.0])
+    w = np.array([5.0, 9.0])
+    uu, vv, ww = O.normalize_uvw(u, v, w)
+    assert abs(uu[0] - 1 / 2002) < 1e-15 and abs(uu[1] - 2.004 / 8.008) < 1e-15
+    assert abs(vv[1] - 1.004 / 8.008) < 1e-15
+    assert ww.tolist() == [0.0, 1.0]
+    # A dominant positive maximum: maxg = 10 * 1.001; x = 10 -> 20.01 / 20.02.
+    u = np.array([10.0, -1.0, 3.0])
+    v = np.array([2.0, 0.5, -6.0])
+    w = np.array([-2.0, 0.0, 2.0])
+    uu, vv, ww = O.normalize_uvw(u, v, w)
+    assert abs(uu[0] - 20.01 / 20.02) < 1e-15 and abs(vv[2] - 4.01 / 20.02) < 1e-15
+    assert ww.tolist() == [0.0, 0.5, 1.0]
+    # Mirror symmetry: x and -x land symmetric about 1/2.
+    u = np.array([7.0, -7.0, 1.5, -1.5])
+    uu, _, _ = O.normalize_uvw(u, u[::-1].copy(), np.array([0.0, 1.0, 2.0, 3.0]))
+    assert abs((uu[0] - 0.5) + (uu[1] - 0.5)) < 1e-15 and abs((uu[2] - 0.5) + (uu[3] - 0.5)) < 1e-15
+    # A minimum above the 1e20 start value is never taken: with every w
+    # above 1e20, wmin stays 1e20.
+    w = np.array([2e20, 3e20])
+    _, _, ww = O.normalize_uvw(np.array([1.0, 2.0]), np.array([1.0, 2.0]), w)
+    assert ww[0] == (2e20 - 1e20) / (3e20 - 1e20)
