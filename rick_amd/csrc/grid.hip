@@ -36,6 +36,7 @@
 // rounding order (identical when F*P == 1).
 #include "rick_internal.h"
 
+#include <atomic>
 #include <type_traits>
 
 #pragma clang fp contract(off)
@@ -64,6 +65,8 @@ constexpr int kBinSlots = 1 << kBinHashBits;
 constexpr int kMaxProbe = 32;
 constexpr uint32_t kZeroTask = 2u;            // task.w of a zero task
 constexpr int kTileThreads = 512;
+constexpr int kLdsBytes = 160 * 1024;         // LDS per CU
+constexpr uint32_t kPersistSlots = 64;        // task slots per resident workgroup to persist
 constexpr int kPadU = 8;  // LDS row padding (elements): rows shift by 8 banks
 
 struct RecF64 {
@@ -315,6 +318,7 @@ k_scan_top(GridGeom g, uint32_t *__restrict__ blocksum, uint32_t nscan,
         meta[2] = carry[2];             // split buckets
         meta[3] = carry[3];             // non-empty buckets
         meta[4] = carry[1];             // accumulation tasks (stored first)
+        meta[5] = 0;                    // next task for the tile kernel's workgroups
     }
 }
 
@@ -549,9 +553,18 @@ __device__ __forceinline__ unsigned long long to_fixed(double x, double scale)
 // any cell that matters to the image) and the tile is rounded to FP32 once
 // when written.  A non-finite bound (NaN/Inf data) falls back to FP64
 // atomics.  SMAX >= support bounds the unrolled footprint loops.
+// Tiles whose LDS accumulators leave room for at most two workgroups per CU
+// run persistent workgroups: the task loop costs about 20 VGPRs, which there
+// costs no occupancy, and saves the per-workgroup launch cost of ~10^4 tasks.
+// Smaller tiles keep one workgroup per task.
+__host__ __device__ constexpr bool tile_persistent(int tu, int tv)
+{
+    return 3 * (2 * tv * (tu + kPadU) * (int)sizeof(double)) > kLdsBytes;
+}
+
 template <int PREC, int TU, int TV, int SMAX, bool HIPREC>
 __global__ void __launch_bounds__(kTileThreads)
-k_tile(GridGeom g, const uint4 *__restrict__ tasks, const uint32_t *__restrict__ meta,
+k_tile(GridGeom g, const uint4 *__restrict__ tasks, uint32_t *__restrict__ meta,
        const typename RecOf<PREC>::type *__restrict__ rec, const double *__restrict__ table,
        void *grid_v, int accumulate)
 {
@@ -562,227 +575,275 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, const uint32_t *__restrict__
     __shared__ T stab[kMaxTable];
     double *sre = lds, *sim = lds + TV * LDU;
 
-    // Workgroup i runs accumulation task f(i) when f(i+1) > f(i), where
-    // f(i) = floor(i * Tr / T), else zero task Tr + i - f(i): the two kinds are
-    // spread evenly over the launch so that clearing overlaps accumulation.
+    // Workgroup k runs slot k; persistent workgroups (tile_persistent) then
+    // take slots gridDim.x + meta[5]++ until all are done.  Slot i is
+    // accumulation task f(i) when f(i+1) > f(i), where f(i) = floor(i * Tr / T),
+    // else zero task Tr + i - f(i): the two kinds are spread evenly so that
+    // clearing overlaps accumulation.
     const uint32_t ntask = meta[1], nreal = meta[4];
-    if (blockIdx.x >= ntask) return;
-    const uint32_t fi = (uint32_t)((uint64_t)blockIdx.x * nreal / ntask);
-    const uint32_t fn = (uint32_t)((uint64_t)(blockIdx.x + 1) * nreal / ntask);
-    const uint4 task = tasks[fn > fi ? fi : nreal + blockIdx.x - fi];
-    const uint32_t b = task.x, r0 = task.y, r1 = task.z;
-    const bool split = task.w == 1u;
+    __shared__ uint32_t slot_sh;
+    uint32_t slot = blockIdx.x;
+    for (bool first = true;; first = false) {
+        if (!first) {
+            if (!tile_persistent(TU, TV)) break;
+            // The barrier before the store also orders the previous task's LDS
+            // reads before the next task clears the tile.
+            __syncthreads();
+            if (threadIdx.x == 0) slot_sh = gridDim.x + atomicAdd(&meta[5], 1u);
+            __syncthreads();
+            // Uniform by construction; readfirstlane keeps the task fields and
+            // everything derived from them in scalar registers.
+            slot = __builtin_amdgcn_readfirstlane(slot_sh);
+        }
+        if (slot >= ntask) break;
+        const uint32_t fi = (uint32_t)((uint64_t)slot * nreal / ntask);
+        const uint32_t fn = (uint32_t)((uint64_t)(slot + 1) * nreal / ntask);
+        const uint4 task = tasks[fn > fi ? fi : nreal + slot - fi];
+        const uint32_t b = task.x, r0 = task.y, r1 = task.z;
+        const bool split = task.w == 1u;
 
-    const uint32_t per_plane = (uint32_t)g.tiles_u * (uint32_t)g.tiles_v;
-    const int w = (int)(b / per_plane), rem = (int)(b % per_plane);
-    const int tv = rem / g.tiles_u, tu = rem % g.tiles_u;
-    const int j0 = tu * TU, k0 = tv * TV;
-    const int nj = min(TU, g.nx - j0), nk = min(TV, g.ny - k0);
-    T *grid = static_cast<T *>(grid_v);
+        const uint32_t per_plane = (uint32_t)g.tiles_u * (uint32_t)g.tiles_v;
+        const int w = (int)(b / per_plane), rem = (int)(b % per_plane);
+        const int tv = rem / g.tiles_u, tu = rem % g.tiles_u;
+        const int j0 = tu * TU, k0 = tv * TV;
+        const int nj = min(TU, g.nx - j0), nk = min(TV, g.ny - k0);
+        T *grid = static_cast<T *>(grid_v);
 
-    if (task.w == kZeroTask) {
-        // A run of task.y empty buckets from b, overwrite mode: clear their tiles.
-        T2 z;
-        z.x = T(0);
-        z.y = T(0);
-        T2 *g2 = reinterpret_cast<T2 *>(grid);
-        // FP32 cells are 8 bytes: clear two per 16-byte store where the grid
-        // allows (16-byte aligned base, even row length and tile width).
-        const bool pairs = PREC == RICK_FP32 && (g.nx & 1) == 0 &&
-                           (reinterpret_cast<uintptr_t>(grid) & 15u) == 0u;
-        for (uint32_t e = 0; e < r0; ++e) {
-            const uint32_t be = b + e;
-            const int we = (int)(be / per_plane), re = (int)(be % per_plane);
-            const int je = (re % g.tiles_u) * TU, ke = (re / g.tiles_u) * TV;
-            const int nje = min(TU, g.nx - je), nke = min(TV, g.ny - ke);
-            T2 *t0 = g2 + ((size_t)we * g.ny + (size_t)ke) * (size_t)g.nx + (size_t)je;
-            if (pairs && (nje & 1) == 0) {
-                float4 *t4 = reinterpret_cast<float4 *>(t0);
-                const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                constexpr int HU = TU / 2;
-                for (int idx = threadIdx.x; idx < nke * HU; idx += kTileThreads) {
-                    const int kk = idx / HU, jj = idx % HU;
-                    if (2 * jj < nje) t4[(size_t)kk * (size_t)(g.nx / 2) + jj] = z4;
+        if (task.w == kZeroTask) {
+            // A run of task.y empty buckets from b, overwrite mode: clear their tiles.
+            T2 z;
+            z.x = T(0);
+            z.y = T(0);
+            T2 *g2 = reinterpret_cast<T2 *>(grid);
+            // FP32 cells are 8 bytes: clear two per 16-byte store where the grid
+            // allows (16-byte aligned base, even row length and tile width).
+            const bool pairs = PREC == RICK_FP32 && (g.nx & 1) == 0 &&
+                               (reinterpret_cast<uintptr_t>(grid) & 15u) == 0u;
+            for (uint32_t e = 0; e < r0; ++e) {
+                const uint32_t be = b + e;
+                const int we = (int)(be / per_plane), re = (int)(be % per_plane);
+                const int je = (re % g.tiles_u) * TU, ke = (re / g.tiles_u) * TV;
+                const int nje = min(TU, g.nx - je), nke = min(TV, g.ny - ke);
+                T2 *t0 = g2 + ((size_t)we * g.ny + (size_t)ke) * (size_t)g.nx + (size_t)je;
+                if (pairs && (nje & 1) == 0) {
+                    float4 *t4 = reinterpret_cast<float4 *>(t0);
+                    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                    constexpr int HU = TU / 2;
+                    for (int idx = threadIdx.x; idx < nke * HU; idx += kTileThreads) {
+                        const int kk = idx / HU, jj = idx % HU;
+                        if (2 * jj < nje) t4[(size_t)kk * (size_t)(g.nx / 2) + jj] = z4;
+                    }
+                    continue;
                 }
-                continue;
+                for (int idx = threadIdx.x; idx < nke * TU; idx += kTileThreads) {
+                    const int kk = idx / TU, jj = idx % TU;
+                    if (jj < nje) t0[(size_t)kk * g.nx + jj] = z;
+                }
             }
-            for (int idx = threadIdx.x; idx < nke * TU; idx += kTileThreads) {
-                const int kk = idx / TU, jj = idx % TU;
-                if (jj < nje) t0[(size_t)kk * g.nx + jj] = z;
-            }
+            continue;
         }
-        return;
-    }
 
-    for (int idx = threadIdx.x; idx < 2 * TV * LDU; idx += kTileThreads) lds[idx] = 0.0;
-    for (int idx = threadIdx.x; idx < g.ntable; idx += kTileThreads) stab[idx] = (T)table[idx];
-    // FP32: fixed-point scale from the bound of the task's cell sums.
-    bool fixed = false;
-    double scale = 1.0;
-    if constexpr (PREC == RICK_FP32) {
-        __shared__ double red[2][kTileThreads / 64];
-        double ar = 0.0, ai = 0.0;
-        for (uint32_t r = r0 + threadIdx.x; r < r1; r += kTileThreads) {
-            ar += fabs((double)rec[r].sre);
-            ai += fabs((double)rec[r].sim);
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            ar += __shfl_xor(ar, o);
-            ai += __shfl_xor(ai, o);
-        }
-        if ((threadIdx.x & 63) == 0) {
-            red[0][threadIdx.x >> 6] = ar;
-            red[1][threadIdx.x >> 6] = ai;
+        for (int idx = threadIdx.x; idx < 2 * TV * LDU; idx += kTileThreads) lds[idx] = 0.0;
+        for (int idx = threadIdx.x; idx < g.ntable; idx += kTileThreads) stab[idx] = (T)table[idx];
+        // FP32: fixed-point scale from the bound of the task's cell sums.
+        bool fixed = false;
+        double scale = 1.0;
+        if constexpr (PREC == RICK_FP32) {
+            __shared__ double red[2][kTileThreads / 64];
+            double ar = 0.0, ai = 0.0;
+            for (uint32_t r = r0 + threadIdx.x; r < r1; r += kTileThreads) {
+                ar += fabs((double)rec[r].sre);
+                ai += fabs((double)rec[r].sim);
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                ar += __shfl_xor(ar, o);
+                ai += __shfl_xor(ai, o);
+            }
+            if ((threadIdx.x & 63) == 0) {
+                red[0][threadIdx.x >> 6] = ar;
+                red[1][threadIdx.x >> 6] = ai;
+            }
+            __syncthreads();
+            double br = 0.0, bi = 0.0;
+    #pragma unroll
+            for (int q = 0; q < kTileThreads / 64; ++q) {
+                br += red[0][q];
+                bi += red[1][q];
+            }
+            const double bound = fmax(br, bi) * g.cmax * 1.01;  // margin for FP32 products
+            fixed = isfinite(br) && isfinite(bi) && isfinite(bound);  // (fmax drops NaNs)
+            if (fixed && bound > 0.0) {
+                int e;
+                frexp(bound, &e);                    // bound < 2^e
+                scale = ldexp(1.0, 50 - e);          // bound * scale < 2^50
+            }
         }
         __syncthreads();
-        double br = 0.0, bi = 0.0;
-#pragma unroll
-        for (int q = 0; q < kTileThreads / 64; ++q) {
-            br += red[0][q];
-            bi += red[1][q];
-        }
-        const double bound = fmax(br, bi) * g.cmax * 1.01;  // margin for FP32 products
-        fixed = isfinite(br) && isfinite(bi) && isfinite(bound);  // (fmax drops NaNs)
-        if (fixed && bound > 0.0) {
-            int e;
-            frexp(bound, &e);                    // bound < 2^e
-            scale = ldexp(1.0, 50 - e);          // bound * scale < 2^50
-        }
-    }
-    __syncthreads();
 
-    const int S = g.support, KL = g.kl;
-    // Per-cell Gaussian (GAUSS_HI_PRECISION) or table product: a template
-    // parameter, so each instantiation carries only its own cell arithmetic.
-    constexpr bool hiprec = HIPREC;
-    // gauss_kernel_norm constants (w-stacking.cu:254-256).
-    const double std22 = 1.0 / (2.0 * 1.0 * 1.0);
-    const double gnorm = std22 / kRefPi;
-    // Cells of this tile a footprint may touch, intersected with the grid.
-    const int jlo = j0, jhi = j0 + nj - 1, klo = k0, khi = k0 + nk - 1;
+        const int S = g.support, KL = g.kl;
+        // Per-cell Gaussian (GAUSS_HI_PRECISION) or table product: a template
+        // parameter, so each instantiation carries only its own cell arithmetic.
+        constexpr bool hiprec = HIPREC;
+        // gauss_kernel_norm constants (w-stacking.cu:254-256).
+        const double std22 = 1.0 / (2.0 * 1.0 * 1.0);
+        const double gnorm = std22 / kRefPi;
+        // Cells of this tile a footprint may touch, intersected with the grid.
+        const int jlo = j0, jhi = j0 + nj - 1, klo = k0, khi = k0 + nk - 1;
 
-    // Record r0 + c + 8*lane + wave of each chunk c of 512: consecutive records
-    // (often one baseline at consecutive times, i.e. the same cells) go to
-    // different waves, so the lanes of one atomic instruction rarely collide.
-    const uint32_t roff = (threadIdx.x & 63u) * (kTileThreads / 64) + (threadIdx.x >> 6);
-    // The FP32 fixed-point and the FP64-atomic accumulation are separate
-    // loops, so the hot one carries no code of the other.
-    auto accumulate_all = [&](auto fixed_tag) {
-        constexpr bool FIXED = decltype(fixed_tag)::value;
-        for (uint32_t c0 = r0; c0 < r1; c0 += kTileThreads) {
-            const uint32_t r = c0 + roff;
-            if (r >= r1) continue;
-            const auto rc = rec[r];
-            const double pu = rc.pu, pv = rc.pv;
-            const T s_re = rc.sre, s_im = rc.sim;
-            Foot f;
-            uv_range(g, pu, pv, f);
-            const int ja = max(f.jmin, jlo), jb = min(f.jmax, jhi);
-            const int ka = max(f.kmin, klo), kb = min(f.kmax, khi);
-            const int ju0 = f.gu - KL, kv0 = f.gv - KL;
-            // u-weights of the footprint columns (w-stacking.cu:489, 492).
-            T wu[SMAX];
-            double ud[SMAX];
-            uint32_t umask = 0;
-    #pragma unroll
-            for (int dj = 0; dj < SMAX; ++dj) {
-                const int j = ju0 + dj;
-                wu[dj] = T(0);
-                ud[dj] = 0.0;
-                if (dj < S && j >= ja && j <= jb) {
-                    const double u_dist = (double)j + 0.5 - pu;
-                    ud[dj] = u_dist;
-                    const int jKer = (int)(kOversample * (fabs(u_dist + (double)KL)));
-                    if (hiprec) {
-                        umask |= 1u << dj;
-                    } else if (jKer >= 0 && jKer < g.ntable) {
-                        wu[dj] = stab[jKer];
-                        umask |= 1u << dj;
-                    }
-                }
-            }
-            if (umask == 0) continue;
-    #pragma unroll
-            for (int dk = 0; dk < SMAX; ++dk) {
-                const int k = kv0 + dk;
-                if (dk >= S || k < ka || k > kb) continue;
-                // w-stacking.cu:484, 493
-                const double v_dist = (double)k + 0.5 - pv;
-                T wv = T(0);
-                if (!hiprec) {
-                    const int kKer = (int)(kOversample * (fabs(v_dist + (double)KL)));
-                    if (kKer < 0 || kKer >= g.ntable) continue;
-                    wv = stab[kKer];
-                }
-                // LDS index of footprint column 0 on this row; only columns with
-                // their umask bit set (inside the tile) are ever accessed.
-                const int rowbase = (k - k0) * LDU + (ju0 - j0);
-    #pragma unroll
+        // Record r0 + c + 8*lane + wave of each chunk c of 512: consecutive records
+        // (often one baseline at consecutive times, i.e. the same cells) go to
+        // different waves, so the lanes of one atomic instruction rarely collide.
+        const uint32_t roff = (threadIdx.x & 63u) * (kTileThreads / 64) + (threadIdx.x >> 6);
+        // The FP32 fixed-point and the FP64-atomic accumulation are separate
+        // loops, so the hot one carries no code of the other.
+        auto accumulate_all = [&](auto fixed_tag) {
+            constexpr bool FIXED = decltype(fixed_tag)::value;
+            for (uint32_t c0 = r0; c0 < r1; c0 += kTileThreads) {
+                const uint32_t r = c0 + roff;
+                if (r >= r1) continue;
+                const auto rc = rec[r];
+                const double pu = rc.pu, pv = rc.pv;
+                const T s_re = rc.sre, s_im = rc.sim;
+                Foot f;
+                uv_range(g, pu, pv, f);
+                const int ja = max(f.jmin, jlo), jb = min(f.jmax, jhi);
+                const int ka = max(f.kmin, klo), kb = min(f.kmax, khi);
+                const int ju0 = f.gu - KL, kv0 = f.gv - KL;
+                // u-weights of the footprint columns (w-stacking.cu:489, 492).
+                T wu[SMAX];
+                double ud[SMAX];
+                uint32_t umask = 0;
+        #pragma unroll
                 for (int dj = 0; dj < SMAX; ++dj) {
-                    if (!((umask >> dj) & 1u)) continue;
-                    // conv_weight = K[jKer] * K[kKer] (w-stacking.cu:499/505), or
-                    // gauss_kernel_norm per cell (w-stacking.cu:496).
-                    const T c = hiprec ? (T)(gnorm * exp(-((ud[dj] * ud[dj]) + (v_dist * v_dist)) *
-                                                         std22))
-                                       : wu[dj] * wv;
-                    const T are = s_re * c, aim = s_im * c;
-                    if constexpr (FIXED) {
-                        __hip_atomic_fetch_add(
-                            reinterpret_cast<unsigned long long *>(&sre[rowbase + dj]),
-                            to_fixed((double)are, scale), __ATOMIC_RELAXED,
-                            __HIP_MEMORY_SCOPE_WORKGROUP);
-                        __hip_atomic_fetch_add(
-                            reinterpret_cast<unsigned long long *>(&sim[rowbase + dj]),
-                            to_fixed((double)aim, scale), __ATOMIC_RELAXED,
-                            __HIP_MEMORY_SCOPE_WORKGROUP);
-                    } else {
-                        __hip_atomic_fetch_add(&sre[rowbase + dj], (double)are, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                        __hip_atomic_fetch_add(&sim[rowbase + dj], (double)aim, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    const int j = ju0 + dj;
+                    wu[dj] = T(0);
+                    ud[dj] = 0.0;
+                    if (dj < S && j >= ja && j <= jb) {
+                        const double u_dist = (double)j + 0.5 - pu;
+                        ud[dj] = u_dist;
+                        const int jKer = (int)(kOversample * (fabs(u_dist + (double)KL)));
+                        if (hiprec) {
+                            umask |= 1u << dj;
+                        } else if (jKer >= 0 && jKer < g.ntable) {
+                            wu[dj] = stab[jKer];
+                            umask |= 1u << dj;
+                        }
+                    }
+                }
+                if (umask == 0) continue;
+        #pragma unroll
+                for (int dk = 0; dk < SMAX; ++dk) {
+                    const int k = kv0 + dk;
+                    if (dk >= S || k < ka || k > kb) continue;
+                    // w-stacking.cu:484, 493
+                    const double v_dist = (double)k + 0.5 - pv;
+                    T wv = T(0);
+                    if (!hiprec) {
+                        const int kKer = (int)(kOversample * (fabs(v_dist + (double)KL)));
+                        if (kKer < 0 || kKer >= g.ntable) continue;
+                        wv = stab[kKer];
+                    }
+                    // LDS index of footprint column 0 on this row; only columns with
+                    // their umask bit set (inside the tile) are ever accessed.
+                    const int rowbase = (k - k0) * LDU + (ju0 - j0);
+        #pragma unroll
+                    for (int dj = 0; dj < SMAX; ++dj) {
+                        if (!((umask >> dj) & 1u)) continue;
+                        // conv_weight = K[jKer] * K[kKer] (w-stacking.cu:499/505), or
+                        // gauss_kernel_norm per cell (w-stacking.cu:496).
+                        const T c =
+                            hiprec ? (T)(gnorm * exp(-((ud[dj] * ud[dj]) + (v_dist * v_dist)) *
+                                                     std22))
+                                   : wu[dj] * wv;
+                        const T are = s_re * c, aim = s_im * c;
+                        if constexpr (FIXED) {
+                            __hip_atomic_fetch_add(
+                                reinterpret_cast<unsigned long long *>(&sre[rowbase + dj]),
+                                to_fixed((double)are, scale), __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_WORKGROUP);
+                            __hip_atomic_fetch_add(
+                                reinterpret_cast<unsigned long long *>(&sim[rowbase + dj]),
+                                to_fixed((double)aim, scale), __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_WORKGROUP);
+                        } else {
+                            __hip_atomic_fetch_add(&sre[rowbase + dj], (double)are,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            __hip_atomic_fetch_add(&sim[rowbase + dj], (double)aim,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
                     }
                 }
             }
+        };
+        if constexpr (PREC == RICK_FP32) {
+            if (fixed) accumulate_all(std::true_type{});
+            else accumulate_all(std::false_type{});
+        } else {
+            accumulate_all(std::false_type{});
         }
-    };
-    if constexpr (PREC == RICK_FP32) {
-        if (fixed) accumulate_all(std::true_type{});
-        else accumulate_all(std::false_type{});
-    } else {
-        accumulate_all(std::false_type{});
-    }
-    __syncthreads();
+        __syncthreads();
 
-    for (int idx = threadIdx.x; idx < nk * TU; idx += kTileThreads) {
-        const int kk = idx / TU, jj = idx % TU;
-        if (jj >= nj) continue;
-        const int li = kk * LDU + jj;
-        T vre, vim;
-        if (PREC == RICK_FP32 && fixed) {
-            const double inv = 1.0 / scale;  // exact: a power of two
-            vre = (T)((double)reinterpret_cast<const long long *>(sre)[li] * inv);
-            vim = (T)((double)reinterpret_cast<const long long *>(sim)[li] * inv);
-        } else {
-            vre = (T)sre[li];
-            vim = (T)sim[li];
-        }
-        const size_t c = ((size_t)w * g.ny + (size_t)(k0 + kk)) * (size_t)g.nx + (size_t)(j0 + jj);
-        if (split) {
-            if (vre != T(0)) unsafeAtomicAdd(&grid[2 * c], vre);
-            if (vim != T(0)) unsafeAtomicAdd(&grid[2 * c + 1], vim);
-        } else {
-            T2 *g2 = reinterpret_cast<T2 *>(grid) + c;
-            T2 v;
-            v.x = vre;
-            v.y = vim;
-            if (accumulate) {
-                const T2 o = *g2;
-                v.x = o.x + v.x;
-                v.y = o.y + v.y;
+        for (int idx = threadIdx.x; idx < nk * TU; idx += kTileThreads) {
+            const int kk = idx / TU, jj = idx % TU;
+            if (jj >= nj) continue;
+            const int li = kk * LDU + jj;
+            T vre, vim;
+            if (PREC == RICK_FP32 && fixed) {
+                const double inv = 1.0 / scale;  // exact: a power of two
+                vre = (T)((double)reinterpret_cast<const long long *>(sre)[li] * inv);
+                vim = (T)((double)reinterpret_cast<const long long *>(sim)[li] * inv);
+            } else {
+                vre = (T)sre[li];
+                vim = (T)sim[li];
             }
-            *g2 = v;
+            const size_t c =
+                ((size_t)w * g.ny + (size_t)(k0 + kk)) * (size_t)g.nx + (size_t)(j0 + jj);
+            if (split) {
+                if (vre != T(0)) unsafeAtomicAdd(&grid[2 * c], vre);
+                if (vim != T(0)) unsafeAtomicAdd(&grid[2 * c + 1], vim);
+            } else {
+                T2 *g2 = reinterpret_cast<T2 *>(grid) + c;
+                T2 v;
+                v.x = vre;
+                v.y = vim;
+                if (accumulate) {
+                    const T2 o = *g2;
+                    v.x = o.x + v.x;
+                    v.y = o.y + v.y;
+                }
+                *g2 = v;
+            }
         }
     }
+}
+
+// Tile grid: one workgroup per task slot, or, for persistent tiles in a
+// launch of at least kPersistSlots task slots per resident workgroup, as many
+// workgroups as the device holds resident at once (occupancy x CUs, cached
+// per device and kernel).  Smaller launches keep one workgroup per slot: they
+// often run beside other jobs on other streams, which persistent workgroups
+// would hold off the CUs.
+template <int PREC, int TU, int TV, int SMAX, bool HIPREC>
+uint32_t tile_grid(uint32_t task_cap)
+{
+    if (!tile_persistent(TU, TV)) return task_cap;
+    const void *kernel = reinterpret_cast<const void *>(&k_tile<PREC, TU, TV, SMAX, HIPREC>);
+    static std::atomic<uint32_t> resident[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return task_cap;
+    uint32_t r = resident[dev].load(std::memory_order_relaxed);
+    if (r == 0) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kTileThreads, 0) !=
+                hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                hipSuccess ||
+            per_cu <= 0 || cus <= 0)
+            return task_cap;
+        r = (uint32_t)per_cu * (uint32_t)cus;
+        resident[dev].store(r, std::memory_order_relaxed);
+    }
+    return task_cap / kPersistSlots >= r ? r : task_cap;
 }
 
 template <int PREC, int SMAX>
@@ -791,16 +852,18 @@ hipError_t launch_tile_s(const GridGeom &g, const GridWork &w, const double *tab
 {
     using Rec = typename RecOf<PREC>::type;
     const Rec *rec = static_cast<const Rec *>(w.records);
-    const dim3 grd(w.task_cap), blk(kTileThreads);
+    const dim3 blk(kTileThreads);
+#define RICK_TILE_LAUNCH(A, B, HI)                                                          \
+    hipLaunchKernelGGL((k_tile<PREC, A, B, SMAX, HI>),                                      \
+                       dim3(tile_grid<PREC, A, B, SMAX, HI>(w.task_cap)), blk, 0, s,        \
+                       g, w.tasks, w.meta, rec, table, grid, accumulate)
 #define RICK_TILE_CASE(A, B)                                                                \
     if (g.tu == A && g.tv == B) {                                                           \
         tm->begin(RICK_STAGE_TILE, s);                                                      \
         if (g.kind == RICK_GAUSS_HI_PRECISION)                                              \
-            hipLaunchKernelGGL((k_tile<PREC, A, B, SMAX, true>), grd, blk, 0, s, g, w.tasks,  \
-                               w.meta, rec, table, grid, accumulate);                       \
+            RICK_TILE_LAUNCH(A, B, true);                                                   \
         else                                                                                \
-            hipLaunchKernelGGL((k_tile<PREC, A, B, SMAX, false>), grd, blk, 0, s, g, w.tasks, \
-                               w.meta, rec, table, grid, accumulate);                       \
+            RICK_TILE_LAUNCH(A, B, false);                                                  \
         tm->end(RICK_STAGE_TILE, s);                                                        \
         return hipGetLastError();                                                           \
     }
@@ -810,6 +873,7 @@ hipError_t launch_tile_s(const GridGeom &g, const GridWork &w, const double *tab
     RICK_TILE_CASE(128, 32)
     RICK_TILE_CASE(32, 64)
 #undef RICK_TILE_CASE
+#undef RICK_TILE_LAUNCH
     return hipErrorInvalidValue;
 }
 

@@ -72,6 +72,39 @@ def test_grid_tiles_and_split_buckets(tile, rows_per_task):
     assert rel_l2(got, z["grid"]) <= FP64_TOL
 
 
+@pytest.mark.parametrize("precision,tile", [(FP64, (128, 32)), (FP32, (64, 64))])
+def test_grid_persistent_tile_launch_matches_oracle(precision, tile):
+    """2048^2 x 40 planes in tiles whose LDS allows two workgroups per CU:
+    40,960 buckets, enough task slots that the tile kernel runs persistent
+    workgroups pulling slots from a counter.  A cluster of 30,000 rows on
+    three planes splits buckets over several tasks.  Two calls in a row: the
+    slot counter must restart with each launch."""
+    rng = np.random.default_rng(77)
+    n_uni, n_hot, nx, planes, F, P = 50000, 30000, 2048, 40, 2, 1
+    uu = np.concatenate([rng.uniform(0, 1, n_uni), rng.uniform(0.40, 0.41, n_hot)])
+    vv = np.concatenate([rng.uniform(0, 1, n_uni), rng.uniform(0.55, 0.56, n_hot)])
+    ww = np.concatenate([rng.uniform(0, 1, n_uni), rng.uniform(0.50, 0.575, n_hot)])
+    n = uu.size
+    vr = rng.normal(size=n * F * P).astype(np.float32)
+    vi = rng.normal(size=n * F * P).astype(np.float32)
+    wt = rng.uniform(0.5, 1.5, n * P).astype(np.float32)
+    ref = O.wstack(O.KAISERBESSEL, planes, uu, vv, ww, vr, vi, wt, F, P, 1.0 / nx, 1.0 / planes,
+                   7, nx, nx, threads=8)
+    print(f"case persistent tiles precision={precision} tile={tile}", flush=True)
+    plan = Plan(nx, nx, planes, kernel=rick_amd.KAISERBESSEL, support=7, precision=precision,
+                max_points=n, tile=tile)
+    ins = [dev(a) for a in (uu, vv, ww, vr, vi, wt)]
+    tol = FP64_TOL if precision == FP64 else 1e-6
+    for call in range(2):
+        got = host(plan.grid(*ins, F, P, out=nan_grid(plan)))
+        assert np.isfinite(got).all()
+        err = rel_l2(got, ref)
+        print(f"call {call}: rel L2 {err:.3e}", flush=True)
+        assert err <= tol
+        del got
+    assert plan.grid_stats()["split_buckets"] > 0
+
+
 @pytest.mark.parametrize("precision", [FP64, FP32])
 @pytest.mark.parametrize("F,P", [(2, 1), (16, 1), (4, 2), (64, 1)])
 def test_grid_scattered_rows_overflow_block_tables(precision, F, P):
