@@ -388,7 +388,75 @@ k_scan_down(GridGeom g, const uint32_t *__restrict__ counts,
     }
 }
 
-template <int PREC, bool VEC4>
+// Channel/polarization sums of the 64 rows r0 .. r0+63 of one wave, read
+// cooperatively when a row holds F*P = 4L floats (L = 2 or 4): the wave's rows
+// are 64*4L contiguous floats, so load j covers rows r0 + j*(64/L) ... with
+// lane l holding elements 4(l%L) .. 4(l%L)+3 of row r0 + j*(64/L) + l/L in one
+// 16-byte read, and every load instruction reads whole lines.  The running
+// sum walks the row's L lanes in element order (shfl_up), so each row's sum is
+// formed in the order of the per-row loop in k_scatter (w-stacking.cu:512-527).
+// Lane l returns the sums of row r0 + l (zero past n).
+template <int L>
+__device__ __forceinline__ void wave_row_sums(uint32_t r0, uint32_t n, uint32_t P,
+                                              const float *__restrict__ vr,
+                                              const float *__restrict__ vi,
+                                              const float *__restrict__ wt, double &out_re,
+                                              double &out_im)
+{
+    static_assert(L == 2 || L == 4, "four or eight lanes' worth of floats per row");
+    constexpr int RPL = 64 / L;  // rows per load
+    const int lane = threadIdx.x & 63;
+    const int q = lane % L;
+    const float4 *r4 = reinterpret_cast<const float4 *>(vr);
+    const float4 *i4 = reinterpret_cast<const float4 *>(vi);
+    out_re = out_im = 0.0;
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        const uint32_t row = r0 + (uint32_t)(j * RPL + lane / L);
+        float4 ra = make_float4(0.f, 0.f, 0.f, 0.f), ia = ra;
+        float wv[4] = {0.f, 0.f, 0.f, 0.f};
+        if (row < n) {
+            const size_t e = (size_t)row * L + (size_t)q;
+            ra = r4[e];
+            ia = i4[e];
+            const float *wrow = wt + (size_t)row * P;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) wv[c] = wrow[P == 1 ? 0u : (uint32_t)(4 * q + c) % P];
+        }
+        const float rr[4] = {ra.x, ra.y, ra.z, ra.w}, ii[4] = {ia.x, ia.y, ia.z, ia.w};
+        double run_re = 0.0, run_im = 0.0;
+#pragma unroll
+        for (int st = 0; st < L; ++st) {
+            if (q == st) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    if (!isnan(rr[c])) {
+                        const float wr = wv[c] * rr[c];
+                        const float wi = wv[c] * ii[c];
+                        run_re += (double)wr;
+                        run_im += (double)wi;
+                    }
+                }
+            }
+            if (st + 1 < L) {
+                run_re = __shfl_up(run_re, 1, L);
+                run_im = __shfl_up(run_im, 1, L);
+            }
+        }
+        // Row j*RPL + m of this load ends in lane m*L + L-1.
+        const int src = (lane % RPL) * L + (L - 1);
+        const double t_re = __shfl(run_re, src), t_im = __shfl(run_im, src);
+        if (lane / RPL == j) {
+            out_re = t_re;
+            out_im = t_im;
+        }
+    }
+}
+
+// VMODE: 0 = per-row scalar reads, 1 = per-row 16-byte reads (F*P a multiple
+// of 4), 2 or 4 = wave-cooperative reads with VMODE lanes per row (F*P = 8 or
+// 16); 16-byte modes need 16-byte aligned visibility arrays.
+template <int PREC, int VMODE>
 __global__ void __launch_bounds__(kScatterThreads)
 k_scatter(GridGeom g, uint32_t n, uint32_t F, uint32_t P, const double *__restrict__ uu,
           const double *__restrict__ vv, const double *__restrict__ ww,
@@ -403,9 +471,8 @@ k_scatter(GridGeom g, uint32_t n, uint32_t F, uint32_t P, const double *__restri
     }
     __syncthreads();
     const uint32_t base = blockIdx.x * kRowsPerBlock;
-    // 16-byte visibility reads (F*P a multiple of 4, aligned arrays): chosen
-    // on the host, so each instantiation carries one read path.
-    constexpr bool vec4 = VEC4;
+    // The read path is chosen on the host, so each instantiation carries one.
+    constexpr bool vec4 = VMODE == 1;
     double pu[kRowsPerThread], pv[kRowsPerThread], sre[kRowsPerThread], sim[kRowsPerThread];
     uint32_t item[kRowsPerThread][4];
     int nk[kRowsPerThread];
@@ -414,6 +481,9 @@ k_scatter(GridGeom g, uint32_t n, uint32_t F, uint32_t P, const double *__restri
         const uint32_t i = base + k * kScatterThreads + threadIdx.x;
         nk[k] = 0;
         pu[k] = pv[k] = sre[k] = sim[k] = 0.0;
+        double w_re = 0.0, w_im = 0.0;
+        if constexpr (VMODE >= 2)  // whole wave, before any lane leaves
+            wave_row_sums<VMODE>(i - (threadIdx.x & 63u), n, P, vr, vi, wt, w_re, w_im);
         Foot f;
         if (i >= n) continue;
         pu[k] = uu[i] / g.dx;
@@ -425,7 +495,10 @@ k_scatter(GridGeom g, uint32_t n, uint32_t F, uint32_t P, const double *__restri
         const size_t vb = (size_t)i * FP;
         const float *wrow = wt + (size_t)i * P;
         double a_re = 0.0, a_im = 0.0;
-        if (vec4) {
+        if constexpr (VMODE >= 2) {
+            a_re = w_re;
+            a_im = w_im;
+        } else if (vec4) {
             // Same sum in the same element order, read 16 bytes at a time.
             const float4 *r4 = reinterpret_cast<const float4 *>(vr + vb);
             const float4 *i4 = reinterpret_cast<const float4 *>(vi + vb);
@@ -980,20 +1053,28 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
     tm->end(RICK_STAGE_SCAN, s);
     tm->begin(RICK_STAGE_SCATTER, s);
     if (n > 0) {
-        const bool vec4 = ((F * P) % 4u) == 0u &&
+        const uint32_t FP = F * P;
+        const bool vec4 = (FP % 4u) == 0u &&
                           ((reinterpret_cast<uintptr_t>(vr) | reinterpret_cast<uintptr_t>(vi)) &
                            15u) == 0u;
-#define RICK_SCATTER(PR, V4)                                                                  \
-    hipLaunchKernelGGL((k_scatter<PR, V4>), dim3(nblk), dim3(kScatterThreads), 0, s, g, n, F, P, \
+        const int mode = !vec4 ? 0 : FP == 8u ? 2 : FP == 16u ? 4 : 1;
+#define RICK_SCATTER(PR, VM)                                                                  \
+    hipLaunchKernelGGL((k_scatter<PR, VM>), dim3(nblk), dim3(kScatterThreads), 0, s, g, n, F, P, \
                        uu, vv, ww, vr, vi, wt, w.cursor,                                      \
                        static_cast<typename RecOf<PR>::type *>(w.records))
+#define RICK_SCATTER_MODES(PR)                                                                \
+    switch (mode) {                                                                           \
+    case 4: RICK_SCATTER(PR, 4); break;                                                       \
+    case 2: RICK_SCATTER(PR, 2); break;                                                       \
+    case 1: RICK_SCATTER(PR, 1); break;                                                       \
+    default: RICK_SCATTER(PR, 0); break;                                                      \
+    }
         if (precision == RICK_FP64) {
-            if (vec4) RICK_SCATTER(RICK_FP64, true);
-            else RICK_SCATTER(RICK_FP64, false);
+            RICK_SCATTER_MODES(RICK_FP64)
         } else {
-            if (vec4) RICK_SCATTER(RICK_FP32, true);
-            else RICK_SCATTER(RICK_FP32, false);
+            RICK_SCATTER_MODES(RICK_FP32)
         }
+#undef RICK_SCATTER_MODES
 #undef RICK_SCATTER
     }
     tm->end(RICK_STAGE_SCATTER, s);
