@@ -19,7 +19,7 @@
 //   k_scatter    forms per row S = sum_{f,p} fl32(weight*vis) (NaN real parts
 //                skipped, w-stacking.cu:517) and writes a record {pos_u,
 //                pos_v, S} to each of its buckets; slots come from one
-//                returning global atomic per distinct key per 4096-row block.
+//                returning global atomic per distinct key per 2048-row block.
 //   k_zero_split clears the tiles of split buckets (overwrite mode).
 //   k_tile       one workgroup per task: accumulates its records into an FP64
 //                LDS tile (one lane per record, LDS atomics), then writes
@@ -47,10 +47,11 @@ namespace {
 constexpr int kThreads = 256;                 // count / scatter / scan workgroups
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int kScanBlock = 4 * kThreads;      // buckets per scan block
-// Scatter: 4096-row blocks of 1024 threads, 4 rows in flight per thread;
+// Scatter: 2048-row blocks of 1024 threads, 2 rows in flight per thread (64
+// VGPRs: two blocks per CU);
 // keys beyond the hash table's reach take a per-record global atomic.
 constexpr int kScatterThreads = 1024;
-constexpr int kRowsPerThread = 4;
+constexpr int kRowsPerThread = 2;
 constexpr int kRowsPerBlock = kScatterThreads * kRowsPerThread;
 constexpr int kHashBits = 12;
 constexpr int kHashSlots = 1 << kHashBits;
@@ -176,7 +177,9 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v)
         p[i] = v;
 }
 
-__global__ void __launch_bounds__(kBinThreads)
+// 8 waves per SIMD: two 1024-thread blocks per CU (their hash tables take
+// 64 KB of LDS each), which needs the SGPR count held to the 8-wave budget.
+__global__ void __launch_bounds__(kBinThreads, 8)
 k_count(GridGeom g, uint32_t n, const double *__restrict__ uu, const double *__restrict__ vv,
         const double *__restrict__ ww, uint32_t *__restrict__ counts)
 {
@@ -187,10 +190,25 @@ k_count(GridGeom g, uint32_t n, const double *__restrict__ uu, const double *__r
     }
     __syncthreads();
     const uint32_t base = blockIdx.x * (uint32_t)kBinRows;
+    // The next row's coordinates are loaded before the current row is hashed,
+    // so their latency overlaps the LDS work.
+    uint32_t i = base + threadIdx.x;
+    double u = 0.0, v = 0.0, w = 0.0;
+    if (i < n) {
+        u = uu[i];
+        v = vv[i];
+        w = ww[i];
+    }
     for (int k = 0; k < kBinRowsPerThread; ++k) {
-        const uint32_t i = base + k * kBinThreads + threadIdx.x;
+        const uint32_t in = i + kBinThreads;
+        double un = 0.0, vn = 0.0, wn = 0.0;
+        if (k + 1 < kBinRowsPerThread && in < n) {
+            un = uu[in];
+            vn = vv[in];
+            wn = ww[in];
+        }
         Foot f;
-        if (i < n && footprint(g, uu[i] / g.dx, vv[i] / g.dx, ww[i] / g.dw, f)) {
+        if (i < n && footprint(g, u / g.dx, v / g.dx, w / g.dw, f)) {
             uint32_t keys[4];
             const int nk = row_keys(g, f, keys);
             for (int q = 0; q < nk; ++q) {
@@ -199,6 +217,10 @@ k_count(GridGeom g, uint32_t n, const double *__restrict__ uu, const double *__r
                 else atomicAdd(&hcnt[h], 1u);
             }
         }
+        i = in;
+        u = un;
+        v = vn;
+        w = wn;
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kBinSlots; i += kBinThreads) {
@@ -457,7 +479,7 @@ __device__ __forceinline__ void wave_row_sums(uint32_t r0, uint32_t n, uint32_t 
 // of 4), 2 or 4 = wave-cooperative reads with VMODE lanes per row (F*P = 8 or
 // 16); 16-byte modes need 16-byte aligned visibility arrays.
 template <int PREC, int VMODE>
-__global__ void __launch_bounds__(kScatterThreads)
+__global__ void __launch_bounds__(kScatterThreads, 8)
 k_scatter(GridGeom g, uint32_t n, uint32_t F, uint32_t P, const double *__restrict__ uu,
           const double *__restrict__ vv, const double *__restrict__ ww,
           const float *__restrict__ vr, const float *__restrict__ vi,
