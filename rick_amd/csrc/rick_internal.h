@@ -79,7 +79,8 @@ struct GridWork {
     uint4 *tasks;           // [task_cap] (bucket, first record, end record, split)
     uint32_t *split_list;   // [split_cap] buckets handled by more than one task
     uint32_t *meta;         // [8]: records, tasks (all), split buckets, non-empty buckets,
-                            //   accumulation tasks (stored before the zero tasks)
+                            //   accumulation tasks (stored before the zero tasks), the
+                            //   tile kernel's next task slot (persistent workgroups)
     void *records;          // [rec_cap] sorted row records
     uint32_t *band;         // [planes*tiles_v] tile rows holding an occupied tile
     uint32_t rec_cap, split_cap, task_cap, nscan;

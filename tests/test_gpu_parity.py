@@ -78,7 +78,9 @@ def test_grid_persistent_tile_launch_matches_oracle(precision, tile):
     40,960 buckets, enough task slots that the tile kernel runs persistent
     workgroups pulling slots from a counter.  A cluster of 30,000 rows on
     three planes splits buckets over several tasks.  Two calls in a row: the
-    slot counter must restart with each launch."""
+    slot counter must restart with each launch.  A third call with no rows
+    runs only zero tasks, far more than the resident workgroups, and must
+    clear the whole grid."""
     rng = np.random.default_rng(77)
     n_uni, n_hot, nx, planes, F, P = 50000, 30000, 2048, 40, 2, 1
     uu = np.concatenate([rng.uniform(0, 1, n_uni), rng.uniform(0.40, 0.41, n_hot)])
@@ -103,6 +105,9 @@ def test_grid_persistent_tile_launch_matches_oracle(precision, tile):
         assert err <= tol
         del got
     assert plan.grid_stats()["split_buckets"] > 0
+    empty = [t[:0] for t in ins]
+    got = plan.grid(*empty, F, P, out=nan_grid(plan))
+    assert not host(got).any()
 
 
 @pytest.mark.parametrize("precision", [FP64, FP32])
