@@ -191,6 +191,11 @@ int rick_plan_create(rick_plan *out, const rick_plan_desc *desc)
         return set_error(RICK_ERR_INVALID, "max_points %u too large (limit 2^30)", d.max_points);
 
     p->grid_ok = d.max_points > 0;
+    if (p->grid_ok) {
+        if (const char *why = grid_launch_limit(g, d.max_points))
+            return set_error(RICK_ERR_INVALID, "%d x %d x %d grid, %d x %d tiles, %u points: %s",
+                             g.nx, g.ny, g.planes, tu, tv, d.max_points, why);
+    }
     p->fft_ok = is_pow2(g.nx) && is_pow2(g.ny) && g.nx >= 16 && g.ny >= 16 && g.nx <= 8192 &&
                 g.ny <= 8192;
 

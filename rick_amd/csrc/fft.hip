@@ -547,7 +547,10 @@ k_fft_cols_phase(ColArgs a, int planes, const C2<T> *__restrict__ tw,
     constexpr int G = lines_per_wg(L);
     constexpr int NT = G * L / kE;
     __shared__ C2<T> lds[lds_elems(L, false)];
-    __shared__ unsigned char rl[256];
+    // Band flags of two planes: with the FP64 prefetch, plane iw + 1's flags
+    // are written while other waves may still be choosing which of plane
+    // iw's rows to load, so consecutive planes use different halves.
+    __shared__ unsigned char rl[2][256];
     __shared__ double nfl[sizeof(T) == 4 ? G * L : 1];
     const int c0 = blockIdx.x * G;
     const int s = blockIdx.y;
@@ -603,7 +606,7 @@ k_fft_cols_phase(ColArgs a, int planes, const C2<T> *__restrict__ tw,
         }
     }
     auto plane_lines = [&](int iw) {
-        if (co.band) col_rowlive(co, iw, s * a.in_set_mul, a.in_stride, rl, ln);
+        if (co.band) col_rowlive(co, iw, s * a.in_set_mul, a.in_stride, rl[iw & 1], ln);
         ln.in_base = (size_t)iw * (size_t)a.nx * (size_t)a.ny +
                      (size_t)s * a.in_set_mul * (size_t)a.nx + c0;
     };

@@ -1000,6 +1000,22 @@ void capacities(const GridGeom &g, uint32_t max_points, uint32_t *rec_cap, uint3
 
 }  // namespace
 
+const char *grid_launch_limit(const GridGeom &g, uint32_t max_points)
+{
+    uint32_t rec_cap, split_cap, task_cap, nscan;
+    capacities(g, max_points, &rec_cap, &split_cap, &task_cap, &nscan);
+    // A launch holds fewer than 2^32 threads.  Persistent tile launches are
+    // capped at the resident workgroups; the others take one workgroup per
+    // task slot, and k_zero_split one per split-bucket slot.
+    constexpr uint64_t kMaxThreads = 1ull << 32;
+    if (!tile_persistent(g.tu, g.tv) && (uint64_t)task_cap * kTileThreads >= kMaxThreads)
+        return "too many gridding tasks for one tile launch (fewer tiles or rows per "
+               "task needed)";
+    if ((uint64_t)split_cap * kThreads >= kMaxThreads)
+        return "too many split buckets for one launch (rows_per_task too small)";
+    return nullptr;
+}
+
 size_t grid_work_bytes(const GridGeom &g, uint32_t max_points, int precision)
 {
     uint32_t rec_cap, split_cap, task_cap, nscan;

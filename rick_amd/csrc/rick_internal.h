@@ -87,6 +87,9 @@ struct GridWork {
 };
 
 size_t grid_work_bytes(const GridGeom &g, uint32_t max_points, int precision);
+// Null if every gridding launch of a plan with this geometry and capacity
+// fits HIP's launch limits, else why not.
+const char *grid_launch_limit(const GridGeom &g, uint32_t max_points);
 void grid_work_carve(const GridGeom &g, uint32_t max_points, int precision, void *base,
                      GridWork *w);
 

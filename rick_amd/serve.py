@@ -5,8 +5,10 @@ arrive over time.  The scheduler keeps up to `slots` jobs on the device at
 once, each on its own stream, and admits queued requests in arrival order as
 soon as a slot and enough HBM are free: a job's footprint (its grid, images and
 plan workspace, bytes computed from planes x grid^2 x element size and the
-gridder's capacities) is charged against `hbm_budget` while it runs.  Plans
-are pooled per job signature, so after warm-up no request allocates.
+gridder's capacities) is charged against `hbm_budget`.  Plans and buffers are
+pooled per job signature, so after warm-up no request allocates; idle pooled
+buffers stay charged to the budget too, and when a job would not fit, idle
+entries of other signatures are freed, least recently used first.
 
 Faults handled without disturbing other jobs:
   * a request whose footprint exceeds the whole budget is rejected at arrival;
@@ -57,6 +59,50 @@ def grid_footprint(cfg, precision) -> int:
     return grid + images + work
 
 
+class PlanPool:
+    """Idle per-signature resources with their budgeted bytes, least recently
+    used first.  The executor returns a finished job's resources here and
+    takes them back for the next job of the same signature."""
+
+    def __init__(self):
+        self.entries = collections.OrderedDict()   # token -> (sig, nbytes, resources)
+        self._next = 0
+
+    @property
+    def nbytes(self) -> int:
+        return sum(e[1] for e in self.entries.values())
+
+    def put(self, sig, nbytes, res):
+        self.entries[self._next] = (sig, nbytes, res)
+        self._next += 1
+
+    def reusable(self, sig) -> int:
+        """Bytes of the idle entry a job of `sig` would take (0: none)."""
+        for s, n, _ in reversed(self.entries.values()):
+            if s == sig:
+                return n
+        return 0
+
+    def take(self, sig):
+        for tok in reversed(self.entries):
+            if self.entries[tok][0] == sig:
+                return self.entries.pop(tok)[2]
+        return None
+
+    def evict(self, nbytes, keep_sig=None) -> int:
+        """Drops idle entries, oldest first, until at least `nbytes` are freed,
+        sparing the entry take(keep_sig) would return; returns the bytes freed."""
+        keep = next((t for t in reversed(self.entries) if self.entries[t][0] == keep_sig), None)
+        freed = 0
+        for tok in list(self.entries):
+            if freed >= nbytes:
+                break
+            if tok == keep:
+                continue
+            freed += self.entries.pop(tok)[1]
+        return freed
+
+
 class GpuExecutor:
     """Runs imaging jobs on a GPU: one stream per slot, pooled plans."""
 
@@ -68,10 +114,22 @@ class GpuExecutor:
         self.device = device
         self.streams = [torch.cuda.Stream(device=device) for _ in range(slots)]
         self.free_streams = list(range(slots))
-        self.pool = collections.defaultdict(list)   # signature -> idle (plan, grid, re, im)
+        self.pool = PlanPool()                       # idle (plan, grid, re, im)
 
     def footprint(self, key) -> int:
         return grid_footprint(self.ds[key].cfg, self.precision)
+
+    def pooled_bytes(self) -> int:
+        return self.pool.nbytes
+
+    def reusable_bytes(self, key) -> int:
+        return self.pool.reusable(self._sig(key))
+
+    def evict(self, nbytes, keep_key=None) -> int:
+        freed = self.pool.evict(nbytes, None if keep_key is None else self._sig(keep_key))
+        if freed:
+            self.torch.cuda.empty_cache()     # grids and images go back to the device
+        return freed
 
     def _sig(self, key):
         ds = self.ds[key]
@@ -80,9 +138,9 @@ class GpuExecutor:
 
     def _acquire(self, key):
         from .imaging import plan_for
-        sig = self._sig(key)
-        if self.pool[sig]:
-            return self.pool[sig].pop()
+        res = self.pool.take(self._sig(key))
+        if res is not None:
+            return res
         torch = self.torch
         ds = self.ds[key]
         plan = plan_for(ds, precision=self.precision)
@@ -92,14 +150,16 @@ class GpuExecutor:
                 torch.empty(npix, dtype=torch.float64, device=self.device))
 
     def start(self, req):
-        """Issue the job; returns a handle.  Raises on launch errors."""
+        """Issue the job; returns a handle.  Raises on launch errors, after
+        returning the stream slot (and any resources taken) for later jobs."""
         torch = self.torch
         sidx = self.free_streams.pop()
-        res = self._acquire(req.key)
-        plan, grid, re, im = res
-        ds = self.ds[req.key]
-        stream = self.streams[sidx]
+        res = None
         try:
+            res = self._acquire(req.key)
+            plan, grid, re, im = res
+            ds = self.ds[req.key]
+            stream = self.streams[sidx]
             with torch.cuda.stream(stream):
                 plan.image(ds.uu, ds.vv, ds.ww, ds.vis_real, ds.vis_img, ds.weight,
                            ds.cfg.n_chan, ds.cfg.n_pol, grid=grid, image_real=re, image_imag=im)
@@ -107,7 +167,8 @@ class GpuExecutor:
                 ev.record(stream)
         except Exception:
             self.free_streams.append(sidx)
-            self.pool[self._sig(req.key)].append(res)
+            if res is not None:
+                self.pool.put(self._sig(req.key), self.footprint(req.key), res)
             raise
         return (sidx, res, ev, req.key)
 
@@ -121,7 +182,7 @@ class GpuExecutor:
         if keep_result:
             out = res[2].cpu().numpy() + 1j * res[3].cpu().numpy()
         self.free_streams.append(sidx)
-        self.pool[self._sig(key)].append(res)
+        self.pool.put(self._sig(key), self.footprint(key), res)
         return out
 
 
@@ -158,6 +219,12 @@ class Server:
             need = self.ex.footprint(req.key)
             if self.used + need > self.budget:
                 break                      # wait for memory (arrival order kept)
+            # Idle pooled buffers hold device memory too: the job's own
+            # signature's entry is reused, others are freed (LRU) if needed.
+            over = self.used + need + self.ex.pooled_bytes() - self.ex.reusable_bytes(req.key) \
+                - self.budget
+            if over > 0:
+                self.ex.evict(over, keep_key=req.key)
             self.queue.popleft()
             try:
                 handle = self.ex.start(req)

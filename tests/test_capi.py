@@ -69,6 +69,16 @@ def test_argument_errors_without_device():
     assert create(dx=0.0) == _lib.RICK_ERR_INVALID
     assert create(tile_u=48, tile_v=48) == _lib.RICK_ERR_INVALID
     assert create(max_points=2 ** 31) == _lib.RICK_ERR_INVALID
+    # Launch limits (fewer than 2^32 threads per launch), checked at plan
+    # creation: 8192^2 x 128 planes in 32x32 tiles is 2^23 buckets, so the
+    # one-workgroup-per-task tile launch would need 2^32 threads.
+    big = dict(num_w_planes=128, grid_size_x=8192, grid_size_y=8192, dx=1 / 8192, dw=1 / 128)
+    assert create(tile_u=32, tile_v=32, **big) == _lib.RICK_ERR_INVALID
+    assert b"tile launch" in L.rick_last_error()
+    assert create(tile_u=32, tile_v=32, **dict(big, num_w_planes=64)) != _lib.RICK_ERR_INVALID
+    # One record per task: up to 2 * max_points split buckets.
+    assert create(rows_per_task=1, max_points=2 ** 29) == _lib.RICK_ERR_INVALID
+    assert b"split buckets" in L.rick_last_error()
     assert L.rick_plan_grid(None, 0, 1, 1, *([None] * 7), 0, None) == _lib.RICK_ERR_INVALID
 
 

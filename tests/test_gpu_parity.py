@@ -371,3 +371,44 @@ def test_phase_large_w_matches_oracle(precision):
     err = rel_l2(host(gre) + 1j * host(gim), re + 1j * im)
     print(f"large-w phase precision={precision}: rel-L2 {err:.3e}")
     assert err <= (1e-13 if precision == FP64 else 1e-6)
+
+
+@pytest.mark.parametrize("ny", [256, 512])
+@pytest.mark.parametrize("precision", [FP64, FP32])
+def test_chain_band_flags_differ_between_planes(ny, precision):
+    """Image chain with the single fused column pass (ny <= 512): each plane's
+    rows occupy different tile bands and the grid starts as NaN.  The column
+    pass takes each plane's band flags from LDS while the FP64 variant loads
+    the next plane ahead; flags read for the wrong plane would either drop
+    live rows of a plane or let the NaN of never-written rows into the image."""
+    rng = np.random.default_rng(31 + ny)
+    nx, planes, n_per = 256, 6, 3000
+    uu, vv, ww = [], [], []
+    for p in range(planes):
+        uu.append(rng.uniform(0.05, 0.95, n_per))
+        lo = 0.05 + 0.15 * p
+        vv.append(rng.uniform(lo, lo + 0.05, n_per))
+        ww.append((p + rng.uniform(0.1, 0.9, n_per)) / planes)
+    uu, vv, ww = (np.concatenate(a) for a in (uu, vv, ww))
+    n = uu.size
+    vr = rng.normal(size=n).astype(np.float32)
+    vi = rng.normal(size=n).astype(np.float32)
+    wt = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    res, wmin, wmax = 0.4 / max(nx, ny), -200.0, 200.0
+    g = O.wstack(O.KAISERBESSEL, planes, uu, vv, ww, vr, vi, wt, 1, 1, 1.0 / nx, 1.0 / planes, 7,
+                 nx, ny, threads=8)
+    gs = O.fft_planes(planes, nx, ny, g, threads=8)
+    ore, oim = O.phase_correction(gs, nx, ny, planes, res, wmin, wmax, threads=8)
+    ref = ore + 1j * oim
+    print(f"case band flags ny={ny} precision={precision}", flush=True)
+    plan = Plan(nx, ny, planes, kernel=rick_amd.KAISERBESSEL, support=7, dx=1.0 / nx,
+                dw=1.0 / planes, precision=precision, max_points=n, resolution=res, wmin=wmin,
+                wmax=wmax)
+    ins = [dev(a) for a in (uu, vv, ww, vr, vi, wt)]
+    for call in range(2):
+        re, im, _ = plan.image(*ins, 1, 1, grid=nan_grid(plan))
+        got = host(re) + 1j * host(im)
+        assert np.isfinite(got).all()
+        err = rel_l2(got, ref)
+        print(f"call {call}: rel-L2 {err:.3e}", flush=True)
+        assert err <= (FP64_TOL if precision == FP64 else FP32_TOL)

@@ -7,18 +7,36 @@ from rick_amd import serve
 
 
 class FakeExecutor:
-    """Jobs take `dur[key]` seconds of fake-clock time and `mem[key]` bytes."""
+    """Jobs take `dur[key]` seconds of fake-clock time and `mem[key]` bytes;
+    finished jobs' buffers go to an idle pool keyed by job, as on the GPU."""
 
     def __init__(self, clock, dur, mem, fail_keys=()):
         self.clock, self.dur, self.mem, self.fail = clock, dur, mem, set(fail_keys)
         self.started = []
+        self.pool = serve.PlanPool()
+        self.running_bytes = 0
+        self.peak = 0                     # most device bytes held at once
+        self.allocations = 0
 
     def footprint(self, key):
         return self.mem[key]
 
+    def pooled_bytes(self):
+        return self.pool.nbytes
+
+    def reusable_bytes(self, key):
+        return self.pool.reusable(key)
+
+    def evict(self, nbytes, keep_key=None):
+        return self.pool.evict(nbytes, keep_key)
+
     def start(self, req):
         if req.key in self.fail:
             raise ValueError("bad job")
+        if self.pool.take(req.key) is None:
+            self.allocations += 1
+        self.running_bytes += self.mem[req.key]
+        self.peak = max(self.peak, self.running_bytes + self.pool.nbytes)
         self.started.append(req.rid)
         return (req.key, self.clock.t + self.dur[req.key])
 
@@ -26,7 +44,10 @@ class FakeExecutor:
         return self.clock.t >= handle[1]
 
     def finish(self, handle, keep_result=False):
-        return ("image", handle[0]) if keep_result else None
+        key = handle[0]
+        self.running_bytes -= self.mem[key]
+        self.pool.put(key, self.mem[key], ("buffers", key))
+        return ("image", key) if keep_result else None
 
 
 class FakeClock:
@@ -115,3 +136,52 @@ def test_poisson_requests_rate():
     assert 850 < len(reqs) < 1150
     arr = [r.arrival for r in reqs]
     assert arr == sorted(arr) and arr[-1] <= 20.0
+
+
+def test_running_plus_pooled_bytes_stay_within_budget():
+    """Six job kinds of 30-60 bytes against a 100-byte budget: finished jobs'
+    buffers stay pooled, so admitting new kinds must free idle entries."""
+    mem = {k: 30 + 6 * k for k in range(6)}
+    dur = {k: 0.01 for k in range(6)}
+    reqs = [serve.Request(i, (i * 7) % 6, 0.003 * i) for i in range(60)]
+    srv, ex = run(reqs, dur, mem, 100, 3)
+    assert all(r.status == "done" for r in reqs)
+    assert ex.peak <= 100
+    assert ex.pool.nbytes <= 100
+
+
+def test_pool_reuses_same_job_without_allocating():
+    reqs = [serve.Request(i, 0, 0.02 * i) for i in range(10)]
+    srv, ex = run(reqs, {0: 0.005}, {0: 40}, 100, 2)
+    assert all(r.status == "done" for r in reqs)
+    assert ex.allocations == 1
+
+
+def test_plan_pool_evicts_oldest_and_spares_the_reused_entry():
+    p = serve.PlanPool()
+    p.put("a", 10, 1)
+    p.put("b", 20, 2)
+    p.put("a", 10, 3)
+    assert p.nbytes == 40 and p.reusable("a") == 10 and p.reusable("c") == 0
+    assert p.evict(25, keep_sig="a") == 30          # oldest "a", then "b"
+    assert p.take("a") == 3 and p.nbytes == 0
+
+
+def test_gpu_executor_returns_slot_when_acquire_fails():
+    """A failure while taking a job's plan and buffers (e.g. device out of
+    memory) must hand the stream slot back: later jobs still start."""
+    ex = serve.GpuExecutor.__new__(serve.GpuExecutor)
+    ex.torch = None
+    ex.free_streams = [0, 1]
+    ex.pool = serve.PlanPool()
+
+    def boom(key):
+        raise MemoryError("hipMalloc failed")
+
+    ex._acquire = boom
+    ex._sig = lambda key: key
+    ex.footprint = lambda key: 1
+    for _ in range(3):
+        with pytest.raises(MemoryError):
+            ex.start(serve.Request(0, 0, 0.0))
+    assert sorted(ex.free_streams) == [0, 1]
