@@ -67,9 +67,11 @@ void wstack(int num_w_planes, unsigned int num_points, unsigned int freq_per_cha
 /* Replaces phase_correction(), w-stacking.h:70-83 / phase_correction.cu:97-328.
  * gridss: host, 2*num_w_planes*xaxis*yaxis doubles; image_real/image_imag:
  * host, xaxis*yaxis doubles each, accumulated into.  PHASE_ON behaviour is
- * set by rick_set_phase() (default on, Makefile:56).  Single rank only:
- * xaxis == xaxistot and yaxis == yaxistot.  wmax == wmin is rejected (the
- * reference divides 0/0, phase_correction.cu:102). */
+ * set by rick_set_phase() (default on, Makefile:56).  A rank's slab
+ * (xaxis x yaxis of xaxistot x yaxistot, fourier_transform.c:215) takes its
+ * pixel coordinates as the reference does: local iu, iv against the totals
+ * (phase_correction.cu:204-209).  wmax == wmin is rejected (the reference
+ * divides 0/0, phase_correction.cu:102). */
 void phase_correction(double *gridss, double *image_real, double *image_imag,
                       int xaxis, int yaxis, int num_w_planes, int xaxistot,
                       int yaxistot, double resolution, double wmin, double wmax,
@@ -119,6 +121,11 @@ typedef struct {
     int tile_u, tile_v;
     int rows_per_task;
     int device;             /* HIP device ordinal the plan lives on */
+    /* Phase stage of one rank's slab: the image is grid_size_x x grid_size_y
+     * pixels of an xaxistot x yaxistot image, pixel coordinates taken as the
+     * reference does, local index against the totals (phase_correction.cu:
+     * 204-209).  0 = the plan's own grid size. */
+    int xaxistot, yaxistot;
 } rick_plan_desc;
 
 int rick_plan_create(rick_plan *plan, const rick_plan_desc *desc);

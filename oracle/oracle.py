@@ -259,11 +259,13 @@ def ref_wstack_into(kind, planes, uu, vv, ww, vis_real, vis_img, weight, freq_pe
 
 
 def ref_phase_correction(gridss, nx, ny, planes, resolution, wmin, wmax, phase_on=True,
-                         threads=1):
+                         threads=1, xtot=None, ytot=None):
+    """The reference's phase_correction(); xtot/ytot are its xaxistot/yaxistot
+    (a rank's slab of nx x ny pixels out of xtot x ytot), default nx/ny."""
     gridss = _c(gridss, np.float64)
     re = np.zeros(nx * ny, dtype=np.float64)
     im = np.zeros(nx * ny, dtype=np.float64)
     _ref("phase_on" if phase_on else "phase_off").phase_correction(
-        _ptr(gridss, _dp), _ptr(re, _dp), _ptr(im, _dp), nx, ny, planes, nx, ny, resolution,
-        wmin, wmax, threads, 0)
+        _ptr(gridss, _dp), _ptr(re, _dp), _ptr(im, _dp), nx, ny, planes, xtot or nx, ytot or ny,
+        resolution, wmin, wmax, threads, 0)
     return re, im

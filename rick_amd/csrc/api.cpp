@@ -157,6 +157,9 @@ int rick_plan_create(rick_plan *out, const rick_plan_desc *desc)
         return set_error(RICK_ERR_INVALID, "unknown precision %d", d.precision);
     if (!(d.dx > 0.0) || !(d.dw > 0.0))
         return set_error(RICK_ERR_INVALID, "dx and dw must be positive");
+    if (d.xaxistot < 0 || d.yaxistot < 0)
+        return set_error(RICK_ERR_INVALID, "negative image totals %d x %d", d.xaxistot,
+                         d.yaxistot);
 
     auto p = std::make_unique<rick_plan_s>();
     p->d = d;
@@ -249,8 +252,8 @@ int rick_plan_create(rick_plan *out, const rick_plan_desc *desc)
     if (e == hipSuccess) e = put_tw(g.nx, o_twx);
     if (e == hipSuccess) e = put_tw(g.ny, o_twy);
     std::vector<double> lt(g.nx), mt(g.ny);
-    pixel_sines(g.nx, g.nx, d.resolution, lt.data());
-    pixel_sines(g.ny, g.ny, d.resolution, mt.data());
+    pixel_sines(g.nx, d.xaxistot > 0 ? d.xaxistot : g.nx, d.resolution, lt.data());
+    pixel_sines(g.ny, d.yaxistot > 0 ? d.yaxistot : g.ny, d.resolution, mt.data());
     if (e == hipSuccess)
         e = hipMemcpy(base + o_l, lt.data(), sizeof(double) * g.nx, hipMemcpyHostToDevice);
     if (e == hipSuccess)
@@ -489,7 +492,8 @@ bool same_plan(const rick_plan p, const rick_plan_desc &d)
            a.dx == d.dx && a.dw == d.dw && a.precision == d.precision &&
            a.max_points >= d.max_points && a.resolution == d.resolution && a.wmin == d.wmin &&
            a.wmax == d.wmax && a.phase_on == d.phase_on && a.device == d.device &&
-           a.tile_u == d.tile_u && a.tile_v == d.tile_v && a.rows_per_task == d.rows_per_task;
+           a.tile_u == d.tile_u && a.tile_v == d.tile_v && a.rows_per_task == d.rows_per_task &&
+           a.xaxistot == d.xaxistot && a.yaxistot == d.yaxistot;
 }
 
 int ensure_plan(rick_plan *slot, const rick_plan_desc &d)
@@ -632,12 +636,9 @@ void phase_correction(double *gridss, double *image_real, double *image_imag, in
 {
     clear_error();
     auto body = [&]() -> int {
-        if (!gridss || !image_real || !image_imag || xaxis < 1 || yaxis < 1 || num_w_planes < 1)
+        if (!gridss || !image_real || !image_imag || xaxis < 1 || yaxis < 1 ||
+            num_w_planes < 1 || xaxistot < 1 || yaxistot < 1)
             return set_error(RICK_ERR_INVALID, "bad phase_correction arguments");
-        if (xaxis != xaxistot || yaxis != yaxistot)
-            return set_error(RICK_ERR_UNSUPPORTED,
-                             "multi-rank slabs (xaxis %d != %d or yaxis %d != %d) are not "
-                             "supported", xaxis, xaxistot, yaxis, yaxistot);
         Ctx *c = context_for(rank);
         if (!c) return g_status;
         std::lock_guard<std::mutex> lk(c->mu);
@@ -656,6 +657,8 @@ void phase_correction(double *gridss, double *image_real, double *image_imag, in
         d.wmax = wmax;
         d.phase_on = g_phase_on;
         d.device = c->device;
+        d.xaxistot = xaxistot;
+        d.yaxistot = yaxistot;
         int st = ensure_plan(&c->phase_plan, d);
         if (st != RICK_OK) return st;
         const size_t gbytes = rick_plan_grid_bytes(c->phase_plan);

@@ -6,7 +6,7 @@ phase_correction.cu by ``make -C oracle ref`` (oracle/_ref/), run with one
 thread so the atomic accumulation order is the sequential row order.  Inputs
 are seeded; each .npz holds the inputs and the reference's outputs.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [wstack|phase|phase_slab|chain ...]
 
 Fixtures:
   wstack_<kernel>_s<support>.npz   kernel in gauss/kb/gausshp, support 7 and 9.
@@ -15,6 +15,8 @@ Fixtures:
       polarizations, NaN real parts, rows at w = 1.0 (dropped), rows at the
       grid edges and rows placed on table-bin edges (5*|u_dist+KL| integral).
   phase_<on|off>_p<planes>.npz     32 x 24 planes, PHASE_ON on/off, 1 and 4 planes.
+  phase_on_slab.npz                one rank's slab: 32 x 12 pixels of 36 x 48
+      (xaxis != xaxistot, yaxis != yaxistot), 4 planes, PHASE_ON.
   chain_tiny.npz                   a small synthetic observation through
       reference wstack -> CPU FFT (FFTW absent) -> reference phase_correction.
 """
@@ -94,6 +96,18 @@ def make_phase():
                                 phase_on=on, gridss=gridss, image_real=re, image_imag=im)
 
 
+def make_phase_slab():
+    nx, ny, xtot, ytot, planes = 32, 12, 36, 48, 4
+    rng = np.random.default_rng(91)
+    gridss = rng.normal(size=2 * planes * nx * ny)
+    resolution, wmin, wmax = 1.0 / 500.0, -80.0, 260.0
+    re, im = O.ref_phase_correction(gridss, nx, ny, planes, resolution, wmin, wmax,
+                                    phase_on=True, threads=1, xtot=xtot, ytot=ytot)
+    np.savez_compressed(HERE / "phase_on_slab.npz", nx=nx, ny=ny, xtot=xtot, ytot=ytot,
+                        planes=planes, resolution=resolution, wmin=wmin, wmax=wmax,
+                        phase_on=True, gridss=gridss, image_real=re, image_imag=im)
+
+
 TINY = synth.ObsConfig("tiny", "disk", 16, 1500.0, 45.0, 48, -3.0, 3.0, 45.0, 150e6, 152e6,
                        2, 1, 3, 64, 4, synth.KAISERBESSEL, 7, seed=5)
 
@@ -118,8 +132,9 @@ if __name__ == "__main__":
     if not O.ref_available():
         raise SystemExit("oracle/_ref is not built (needs the reference sources): "
                          "make -C oracle ref")
-    make_wstack()
-    make_phase()
-    make_chain()
+    makers = {"wstack": make_wstack, "phase": make_phase, "phase_slab": make_phase_slab,
+              "chain": make_chain}
+    for name in sys.argv[1:] or list(makers):      # e.g. `make_golden.py phase_slab`
+        makers[name]()
     for f in sorted(HERE.glob("*.npz")):
         print(f.name, f.stat().st_size)

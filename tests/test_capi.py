@@ -3,6 +3,7 @@ and rejects bad arguments before touching a device (no GPU needed)."""
 import ctypes
 import re
 import subprocess
+from pathlib import Path
 
 import pytest
 
@@ -80,6 +81,42 @@ def test_argument_errors_without_device():
     assert create(rows_per_task=1, max_points=2 ** 29) == _lib.RICK_ERR_INVALID
     assert b"split buckets" in L.rick_last_error()
     assert L.rick_plan_grid(None, 0, 1, 1, *([None] * 7), 0, None) == _lib.RICK_ERR_INVALID
+
+
+REFERENCE = Path("/root/reference")
+MPI_INCLUDE = Path("/opt/conda/include")
+
+
+@pytest.mark.skipif(not (REFERENCE / "w-stacking.h").exists() or
+                    not (MPI_INCLUDE / "mpi.h").exists(),
+                    reason="needs the reference's w-stacking.h (and the mpi.h it includes)")
+def test_header_agrees_with_reference_prototypes():
+    """include/rick.h and the reference's w-stacking.h in one C translation
+    unit: repeated declarations must be compatible, so wstack() and
+    phase_correction() keep the reference's C signatures.  A planted wrong
+    prototype is rejected, so the check is not vacuous."""
+    cmd = ["gcc", "-std=gnu11", "-fsyntax-only", f"-I{REFERENCE}", f"-I{MPI_INCLUDE}",
+           f"-I{ROOT / 'include'}", str(ROOT / "tests" / "c" / "header_compat.c")]
+    ok = subprocess.run(cmd, capture_output=True, text=True)
+    assert ok.returncode == 0, ok.stderr
+    bad = subprocess.run(cmd + ["-DRICK_PLANT_MISMATCH"], capture_output=True, text=True)
+    assert bad.returncode != 0 and "conflicting types" in bad.stderr
+
+
+def build_c_caller(out: Path) -> Path:
+    """Builds tests/c/dropin_caller.c against include/rick.h, linked with
+    -lrick_hip as a C program of the reference would be."""
+    lib_dir = _lib.LIB_PATH.parent
+    subprocess.run(["gcc", "-O2", "-std=gnu11", f"-I{ROOT / 'include'}",
+                    str(ROOT / "tests" / "c" / "dropin_caller.c"), f"-L{lib_dir}", "-lrick_hip",
+                    f"-Wl,-rpath,{lib_dir}", "-o", str(out)], check=True)
+    return out
+
+
+def test_c_caller_links_with_library(tmp_path):
+    exe = build_c_caller(tmp_path / "dropin_caller")
+    r = subprocess.run([str(exe), "--version"], capture_output=True, text=True, check=True)
+    assert r.stdout.startswith("rick")
 
 
 def test_python_wrappers_validate_arrays():

@@ -268,6 +268,28 @@ def test_dropin_phase_correction_accumulates():
     assert rel_l2(im + 2, z["image_imag"]) <= 1e-14
 
 
+def test_phase_slab_matches_reference_vector():
+    """One rank's slab, 32 x 12 pixels of a 36 x 48 image (the reference's
+    call at fourier_transform.c:215): through the plan API (FP64 and FP32) and
+    through the drop-in phase_correction(), which accumulates."""
+    z = load_golden("phase_on_slab.npz")
+    nx, ny, planes = int(z["nx"]), int(z["ny"]), int(z["planes"])
+    xtot, ytot = int(z["xtot"]), int(z["ytot"])
+    ref = z["image_real"] + 1j * z["image_imag"]
+    for precision, tol in ((FP64, 1e-14), (FP32, 1e-6)):
+        plan = Plan(nx, ny, planes, resolution=float(z["resolution"]), wmin=float(z["wmin"]),
+                    wmax=float(z["wmax"]), precision=precision, xaxistot=xtot, yaxistot=ytot)
+        src = z["gridss"] if precision == FP64 else z["gridss"].astype(np.float32)
+        re, im = plan.phase(dev(src))
+        assert rel_l2(host(re) + 1j * host(im), ref) <= tol
+    re = np.full(nx * ny, 0.5)
+    im = np.zeros(nx * ny)
+    rick_amd.set_phase(True)
+    rick_amd.phase_correction(z["gridss"], re, im, nx, ny, planes, xtot, ytot,
+                              float(z["resolution"]), float(z["wmin"]), float(z["wmax"]), 2, 0)
+    assert rel_l2((re - 0.5) + 1j * im, ref) <= 1e-14
+
+
 def test_phase_rejects_degenerate_w_range():
     plan = Plan(16, 16, 4, wmin=5.0, wmax=5.0)
     with pytest.raises(rick_amd.RickError):
@@ -412,3 +434,31 @@ def test_chain_band_flags_differ_between_planes(ny, precision):
         err = rel_l2(got, ref)
         print(f"call {call}: rel-L2 {err:.3e}", flush=True)
         assert err <= (FP64_TOL if precision == FP64 else FP32_TOL)
+
+
+def test_c_program_calls_dropin_chain(tmp_path):
+    """A C program (tests/c/dropin_caller.c) linked with -lrick_hip calls
+    wstack() -> rick_fft_planes() -> phase_correction() on host arrays; its
+    grid and image match the oracle run on the inputs it wrote."""
+    import subprocess
+    from test_capi import build_c_caller
+    exe = build_c_caller(tmp_path / "dropin_caller")
+    print("case C drop-in caller", flush=True)
+    r = subprocess.run([str(exe), str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    nx, ny, planes, F, P, n = 128, 96, 4, 2, 1, 20000
+    raw = (tmp_path / "inputs.bin").read_bytes()
+    f64 = np.frombuffer(raw, np.float64, 3 * n)
+    f32 = np.frombuffer(raw, np.float32, 2 * n * F * P + n * P, offset=24 * n)
+    uu, vv, ww = f64[:n], f64[n:2 * n], f64[2 * n:]
+    vr, vi, wt = f32[:n * F * P], f32[n * F * P:2 * n * F * P], f32[2 * n * F * P:]
+    grid = np.fromfile(tmp_path / "grid.bin", np.float64)
+    image = np.fromfile(tmp_path / "image.bin", np.float64)
+    ref = O.wstack(O.KAISERBESSEL, planes, uu, vv, ww, vr, vi, wt, F, P, 1.0 / nx, 1.0 / planes,
+                   7, nx, ny, threads=8)
+    gs = O.fft_planes(planes, nx, ny, ref, threads=8)
+    ore, oim = O.phase_correction(gs, nx, ny, planes, 1.0 / (3.0 * nx), -150.0, 250.0, threads=8)
+    e_grid = rel_l2(grid, ref)
+    e_img = rel_l2(image[:nx * ny] + 1j * image[nx * ny:], ore + 1j * oim)
+    print(f"C caller: grid rel-L2 {e_grid:.3e}, image rel-L2 {e_img:.3e}")
+    assert e_grid <= FP64_TOL and e_img <= FP64_TOL

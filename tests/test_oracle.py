@@ -49,6 +49,23 @@ def test_phase_matches_reference_vectors(tag):
     assert np.array_equal(re, z["image_real"]) and np.array_equal(im, z["image_imag"])
 
 
+def test_phase_slab_matches_reference_vector():
+    """One rank's slab (xaxis != xaxistot, yaxis != yaxistot): pixel
+    coordinates use the local iu, iv against the totals (phase_correction.cu:
+    204-209), the gridss stride the local xaxis."""
+    z = load_golden("phase_on_slab.npz")
+    re, im = O.phase_correction(z["gridss"], int(z["nx"]), int(z["ny"]), int(z["planes"]),
+                                float(z["resolution"]), float(z["wmin"]), float(z["wmax"]),
+                                phase_on=True, threads=1, xtot=int(z["xtot"]),
+                                ytot=int(z["ytot"]))
+    assert np.array_equal(re, z["image_real"]) and np.array_equal(im, z["image_imag"])
+    # The totals matter: the same slab as a whole image differs.
+    re2, _ = O.phase_correction(z["gridss"], int(z["nx"]), int(z["ny"]), int(z["planes"]),
+                                float(z["resolution"]), float(z["wmin"]), float(z["wmax"]),
+                                phase_on=True, threads=1)
+    assert not np.allclose(re2, z["image_real"])
+
+
 def test_chain_tiny_matches_reference_vectors():
     z = load_golden("chain_tiny.npz")
     g = run_oracle_wstack(z)
