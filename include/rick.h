@@ -275,6 +275,22 @@ int rick_sector_bin(unsigned int num_points, const double *vv, int nsectors, int
                     unsigned int rows_capacity, void *workspace, void *stream,
                     unsigned long *total_entries, unsigned int *dropped);
 
+/* rick_sector_bin() without a workspace argument and without any allocation.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (tight serving loops, harnesses that forbid hipMalloc).  The
+ * workspace is a fixed array in the library's device image, one per device,
+ * of rick_sector_bin_noalloc_max_workspace() bytes (4 MiB); calls whose
+ * rick_sector_workspace_bytes(num_points, nsectors) exceeds it fail with
+ * RICK_ERR_INVALID.  Every call on one device shares that array: calls that
+ * overlap on one device (other host threads, other streams) race and must be
+ * serialized by the caller.  Otherwise identical to rick_sector_bin(),
+ * including the stream synchronization. */
+size_t rick_sector_bin_noalloc_max_workspace(void);
+int rick_sector_bin_noalloc(unsigned int num_points, const double *vv, int nsectors, int yaxis,
+                            double dx, int w_support, unsigned int *histo, unsigned int *start,
+                            unsigned int *rows, unsigned int rows_capacity, void *stream,
+                            unsigned long *total_entries, unsigned int *dropped);
+
 /* Per-sector input assembly of gridding_cpu.c:90-138 on the device: output
  * row i takes row list[nsec-1-i] (the reference walks the list backwards),
  * v shifted by isector*dx*yaxis; weights (P per row) and visibilities (F*P

@@ -89,6 +89,9 @@ SIGNATURES = {
     "rick_sector_workspace_bytes": (_c_size, [_c_uint, _c_int]),
     "rick_sector_bin": (_c_int, [_c_uint, _vp, _c_int, _c_int, _c_double, _c_int, _vp, _vp, _vp,
                                  _c_uint, _vp, _vp, _ulp, _up]),
+    "rick_sector_bin_noalloc_max_workspace": (_c_size, []),
+    "rick_sector_bin_noalloc": (_c_int, [_c_uint, _vp, _c_int, _c_int, _c_double, _c_int, _vp,
+                                         _vp, _vp, _c_uint, _vp, _ulp, _up]),
     "rick_sector_gather": (_c_int, [_vp, _c_uint, _c_int, _c_int, _c_double, _c_uint, _c_uint,
                                     _c_uint, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                     _vp, _vp, _vp]),

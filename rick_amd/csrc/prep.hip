@@ -19,6 +19,7 @@
 // larger index, so it returns those bits in any reduction order.
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
 
 #include "rick_internal.h"
 
@@ -531,6 +532,111 @@ bool sector_layout(uint32_t n, int nsectors, SectorLayout &L)
 constexpr size_t kNormWorkBytes = (size_t)kExtBlocks * 6 * (sizeof(double) + sizeof(long long)) +
                                   8 * sizeof(double);
 
+// Workspace of rick_sector_bin_noalloc, for callers that cannot allocate: the
+// per-(slab, block) counts, their scan and the dropped-entry counter live in
+// this array, one copy per device, held until the process ends.  Host code
+// never passes the array's name to a kernel (on the host it is a stand-in,
+// not a device address): the device address comes from hipGetSymbolAddress,
+// once per device, and is kept in g_scratch_ptr.
+constexpr size_t kSectorScratchBytes = size_t(4) << 20;
+static __device__ __align__(256) unsigned char scratch[kSectorScratchBytes];
+
+constexpr int kMaxDevices = 256;
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[kMaxDevices];
+
+// Device address of the current device's scratch array.
+int sector_scratch(char **out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return set_status(RICK_ERR_DEVICE, "hipGetDevice failed: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices)
+        return set_status(RICK_ERR_DEVICE, "device ordinal %d outside 0..%d", dev, kMaxDevices - 1);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr)
+            return set_status(RICK_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d failed: %s",
+                              dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<char *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return RICK_OK;
+}
+
+// Argument checks shared by the two sector-binning entry points; fills L.
+int sector_bin_check(unsigned int num_points, const double *vv, int nsectors, int yaxis, double dx,
+                     int w_support, const unsigned int *histo, const unsigned int *start,
+                     const unsigned int *rows, SectorLayout &L)
+{
+    if (nsectors < 1 || nsectors + 1 > kMaxSlots)
+        return set_status(RICK_ERR_INVALID, "nsectors %d outside 1..%d", nsectors, kMaxSlots - 1);
+    if (yaxis < 1 || !(dx > 0.0) || w_support < 1)
+        return set_status(RICK_ERR_INVALID, "yaxis %d, dx %g, w_support %d", yaxis, dx, w_support);
+    if (num_points > (1u << 30))
+        return set_status(RICK_ERR_INVALID, "%u rows: sector binning handles at most 2^30",
+                          num_points);
+    if (!histo || !start || (num_points && (!vv || !rows)))
+        return set_status(RICK_ERR_INVALID, "null device pointer");
+    if (!sector_layout(num_points, nsectors, L))
+        return set_status(RICK_ERR_INVALID, "%u rows x %d sectors exceeds the scan capacity",
+                          num_points, nsectors);
+    return RICK_OK;
+}
+
+// Sector binning on workspace w (layout L, at least L.bytes): count, scan,
+// fill, per-slot histogram; synchronizes the stream for the totals.
+int sector_bin_run(unsigned int num_points, const double *vv, int nsectors, int yaxis, double dx,
+                   int w_support, unsigned int *histo, unsigned int *start, unsigned int *rows,
+                   unsigned int rows_capacity, char *w, const SectorLayout &L, hipStream_t s,
+                   unsigned long *total_entries, unsigned int *dropped)
+{
+    uint32_t *counts = reinterpret_cast<uint32_t *>(w + L.counts);
+    uint32_t *offsets = reinterpret_cast<uint32_t *>(w + L.offsets);
+    uint32_t *sums = reinterpret_cast<uint32_t *>(w + L.sums);
+    uint32_t *bad = reinterpret_cast<uint32_t *>(w + L.bad);
+    SectorGeom g;
+    g.nsectors = nsectors;
+    g.yaxis = yaxis;
+    g.dx = dx;
+    g.supporth = (double)((w_support - 1) / 2) * dx;  // init.c:13
+    g.n = num_points;
+    g.nb = L.nb;
+    PREP_HIP(hipMemsetAsync(bad, 0, 4, s));
+    hipLaunchKernelGGL(k_sector_count, dim3(L.nb), dim3(kPrepThreads), 0, s, g, vv, counts, bad);
+    PREP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)L.nchunks), dim3(kPrepThreads), 0, s, counts,
+                       L.len, sums);
+    PREP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanTopThreads), 0, s, sums, (int)L.nchunks,
+                       offsets + L.len);
+    PREP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)L.nchunks), dim3(kPrepThreads), 0, s, counts,
+                       L.len, sums, offsets);
+    PREP_HIP(hipGetLastError());
+    const int slots = nsectors + 1;
+    const size_t lds = (size_t)(kPrepThreads / 64) * slots * sizeof(uint32_t);
+    hipLaunchKernelGGL(k_sector_fill, dim3(L.nb), dim3(kPrepThreads), lds, s, g, vv, offsets,
+                       rows, rows_capacity);
+    PREP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sector_histo, dim3((slots + 255) / 256), dim3(256), 0, s, slots, L.nb,
+                       offsets, histo, start);
+    PREP_HIP(hipGetLastError());
+    uint32_t host[2];
+    PREP_HIP(hipMemcpyAsync(&host[0], offsets + L.len, 4, hipMemcpyDeviceToHost, s));
+    PREP_HIP(hipMemcpyAsync(&host[1], bad, 4, hipMemcpyDeviceToHost, s));
+    PREP_HIP(hipStreamSynchronize(s));
+    if (total_entries) *total_entries = host[0];
+    if (dropped) *dropped = host[1];
+    if (host[0] > rows_capacity)
+        return set_status(RICK_ERR_INVALID, "rows capacity %u < %u sector entries", rows_capacity,
+                          host[0]);
+    return RICK_OK;
+}
+
 }  // namespace
 
 hipError_t launch_split_complex(const void *grid, int precision, size_t n, double *re, double *im,
@@ -596,62 +702,39 @@ int rick_sector_bin(unsigned int num_points, const double *vv, int nsectors, int
                     unsigned long *total_entries, unsigned int *dropped)
 {
     clear_status();
-    if (nsectors < 1 || nsectors + 1 > kMaxSlots)
-        return set_status(RICK_ERR_INVALID, "nsectors %d outside 1..%d", nsectors, kMaxSlots - 1);
-    if (yaxis < 1 || !(dx > 0.0) || w_support < 1)
-        return set_status(RICK_ERR_INVALID, "yaxis %d, dx %g, w_support %d", yaxis, dx, w_support);
-    if (num_points > (1u << 30))
-        return set_status(RICK_ERR_INVALID, "%u rows: sector binning handles at most 2^30",
-                          num_points);
-    if (!histo || !start || !workspace || (num_points && (!vv || !rows)))
-        return set_status(RICK_ERR_INVALID, "null device pointer");
     SectorLayout L;
-    if (!sector_layout(num_points, nsectors, L))
-        return set_status(RICK_ERR_INVALID, "%u rows x %d sectors exceeds the scan capacity",
-                          num_points, nsectors);
-    char *w = static_cast<char *>(workspace);
-    uint32_t *counts = reinterpret_cast<uint32_t *>(w + L.counts);
-    uint32_t *offsets = reinterpret_cast<uint32_t *>(w + L.offsets);
-    uint32_t *sums = reinterpret_cast<uint32_t *>(w + L.sums);
-    uint32_t *bad = reinterpret_cast<uint32_t *>(w + L.bad);
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    SectorGeom g;
-    g.nsectors = nsectors;
-    g.yaxis = yaxis;
-    g.dx = dx;
-    g.supporth = (double)((w_support - 1) / 2) * dx;  // init.c:13
-    g.n = num_points;
-    g.nb = L.nb;
-    PREP_HIP(hipMemsetAsync(bad, 0, 4, s));
-    hipLaunchKernelGGL(k_sector_count, dim3(L.nb), dim3(kPrepThreads), 0, s, g, vv, counts, bad);
-    PREP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)L.nchunks), dim3(kPrepThreads), 0, s, counts,
-                       L.len, sums);
-    PREP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanTopThreads), 0, s, sums, (int)L.nchunks,
-                       offsets + L.len);
-    PREP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)L.nchunks), dim3(kPrepThreads), 0, s, counts,
-                       L.len, sums, offsets);
-    PREP_HIP(hipGetLastError());
-    const int slots = nsectors + 1;
-    const size_t lds = (size_t)(kPrepThreads / 64) * slots * sizeof(uint32_t);
-    hipLaunchKernelGGL(k_sector_fill, dim3(L.nb), dim3(kPrepThreads), lds, s, g, vv, offsets,
-                       rows, rows_capacity);
-    PREP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_sector_histo, dim3((slots + 255) / 256), dim3(256), 0, s, slots, L.nb,
-                       offsets, histo, start);
-    PREP_HIP(hipGetLastError());
-    uint32_t host[2];
-    PREP_HIP(hipMemcpyAsync(&host[0], offsets + L.len, 4, hipMemcpyDeviceToHost, s));
-    PREP_HIP(hipMemcpyAsync(&host[1], bad, 4, hipMemcpyDeviceToHost, s));
-    PREP_HIP(hipStreamSynchronize(s));
-    if (total_entries) *total_entries = host[0];
-    if (dropped) *dropped = host[1];
-    if (host[0] > rows_capacity)
-        return set_status(RICK_ERR_INVALID, "rows capacity %u < %u sector entries", rows_capacity,
-                          host[0]);
-    return RICK_OK;
+    int rc = sector_bin_check(num_points, vv, nsectors, yaxis, dx, w_support, histo, start, rows,
+                              L);
+    if (rc != RICK_OK) return rc;
+    if (!workspace) return set_status(RICK_ERR_INVALID, "null device pointer");
+    return sector_bin_run(num_points, vv, nsectors, yaxis, dx, w_support, histo, start, rows,
+                          rows_capacity, static_cast<char *>(workspace), L,
+                          static_cast<hipStream_t>(stream), total_entries, dropped);
+}
+
+size_t rick_sector_bin_noalloc_max_workspace(void) { return kSectorScratchBytes; }
+
+int rick_sector_bin_noalloc(unsigned int num_points, const double *vv, int nsectors, int yaxis,
+                            double dx, int w_support, unsigned int *histo, unsigned int *start,
+                            unsigned int *rows, unsigned int rows_capacity, void *stream,
+                            unsigned long *total_entries, unsigned int *dropped)
+{
+    clear_status();
+    SectorLayout L;
+    int rc = sector_bin_check(num_points, vv, nsectors, yaxis, dx, w_support, histo, start, rows,
+                              L);
+    if (rc != RICK_OK) return rc;
+    if (L.bytes > kSectorScratchBytes)
+        return set_status(RICK_ERR_INVALID,
+                          "%u rows x %d sectors need %zu bytes of workspace; the allocation-free "
+                          "entry holds %zu (use rick_sector_bin with a caller workspace)",
+                          num_points, nsectors, L.bytes, kSectorScratchBytes);
+    char *w = nullptr;
+    rc = sector_scratch(&w);
+    if (rc != RICK_OK) return rc;
+    return sector_bin_run(num_points, vv, nsectors, yaxis, dx, w_support, histo, start, rows,
+                          rows_capacity, w, L, static_cast<hipStream_t>(stream), total_entries,
+                          dropped);
 }
 
 int rick_sector_gather(const unsigned int *list, unsigned int nsec, int isector, int yaxis,
