@@ -14,9 +14,19 @@ FP32 path's gridded visibilities per second.  The whole image chain
 (grid -> per-plane FFT -> phase correction) is timed beside it, and the FP32
 image is compared with the FP64 one.  Per-kernel durations come from HIP
 events on the launch stream (Plan.profile); the roofline object describes the
-dominant gridding kernel.  The CPU baseline is the reference's own wstack()
-(compiled from its sources into oracle/_ref) with OpenMP on this host's cores
-over a bounded sample of the same rows.
+dominant gridding kernel.
+
+Beside the step, in the same JSON line:
+  * ``batch``: 96 independent imaging jobs of uneven size, one HIP graph per
+    precision, against the same jobs imaged one after another in FP64;
+  * ``serve``: the continuous-batching service under Poisson arrivals (the
+    batched job mix plus 10 % LOFAR-like 20M-visibility jobs): p50/p95/p99
+    latency and images/s at a stated rate, the saturated rate, and the
+    highest rate of a doubling sweep whose p99 stays bounded;
+  * ``cpu_baseline``: the reference's own wstack() (compiled from its sources
+    into oracle/_ref) with OpenMP on this host's cores over a bounded sample
+    of the same rows, with all threads and with one, plus the FFT and phase
+    stages on a bounded number of w-planes, giving a CPU image time.
 """
 from __future__ import annotations
 
@@ -56,6 +66,12 @@ def parse():
                     help="also time a binMS write and read-back of the workload under DIR")
     ap.add_argument("--no-dropin", action="store_true",
                     help="skip the host-pointer wstack() drop-in timing (PCIe-inclusive)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched-imaging leg (96 jobs, one HIP graph)")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving leg (continuous batching, latency percentiles)")
+    ap.add_argument("--serve-seconds", type=float, default=8.0,
+                    help="seconds of Poisson arrivals per serving run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one 1 GiB device-to-device copy (known HBM bytes) so that "
                          "profiler byte counters can be calibrated against it")
@@ -90,11 +106,12 @@ def timed(fn, steps, torch):
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
-def run_path(ds, ins, precision, steps, warmup, torch, tile=(0, 0), rpt=0):
+def run_path(ds, ins, precision, steps, warmup, torch, tile=(0, 0), rpt=0, unroll=0):
     from rick_amd import imaging
     cfg = ds.cfg
     F, P = cfg.n_chan, cfg.n_pol
-    plan = imaging.plan_for(ds, precision=precision, tile=tile, rows_per_task=rpt)
+    plan = imaging.plan_for(ds, precision=precision, tile=tile, rows_per_task=rpt,
+                            unroll=unroll)
     grid = plan.new_grid()
     npix = cfg.grid * cfg.grid
     re = torch.empty(npix, dtype=torch.float64, device="cuda:0")
@@ -131,40 +148,82 @@ def run_path(ds, ins, precision, steps, warmup, torch, tile=(0, 0), rpt=0):
     return plan, grid_ms, image_ms, stages, image_stages, stats, img
 
 
-def cpu_baseline(ds_host_fn, cfg, target_s):
-    """Reference wstack() (oracle/_ref) on a bounded sample of the same rows."""
+def host_cores():
+    """(threads used, CPUs in the affinity set, nproc).  The threads used are
+    the affinity set, capped at the 16 CPUs one GPU's share of the host gives."""
+    aff = len(os.sched_getaffinity(0))
+    return max(1, min(16, aff)), aff, os.cpu_count() or aff
+
+
+def cpu_baseline(ds_host_fn, cfg, target_s, host_grid_fn):
+    """The reference's CPU chain on the host cores: its own wstack() (compiled
+    from its sources into oracle/_ref) on a bounded sample of the same rows,
+    with all threads and with one; the FFT stage with the oracle's transform
+    (FFTW, the reference's, is not installed) and the reference's own
+    phase_correction() on a bounded number of w-planes, scaled to the full
+    plane count.  Image time = gridding of every row + FFT + phase, as
+    timings.dat reports them (result.c:40-47)."""
     from oracle import oracle as O
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    threads, aff, nproc = host_cores()
     kind = "reference" if O.ref_available() else "port"
     nx = cfg.grid
     grid = np.zeros(2 * nx * nx * (cfg.planes + 1), dtype=np.float64)
 
-    def run(n):
+    def run(n, th):
         uu, vv, ww, vr, vi, wt = ds_host_fn(n)
         t0 = time.perf_counter()
         if kind == "reference":
             O.ref_wstack_into(cfg.kernel, cfg.planes, uu, vv, ww, vr, vi, wt, cfg.n_chan,
-                              cfg.n_pol, 1.0 / nx, 1.0 / cfg.planes, cfg.support, nx, nx, grid,
-                              threads)
+                              cfg.n_pol, 1.0 / nx, 1.0 / cfg.planes, cfg.support, nx, nx, grid, th)
         else:
             O.wstack(cfg.kernel, cfg.planes, uu, vv, ww, vr, vi, wt, cfg.n_chan, cfg.n_pol,
-                     1.0 / nx, 1.0 / cfg.planes, cfg.support, nx, nx, grid=grid, threads=threads)
+                     1.0 / nx, 1.0 / cfg.planes, cfg.support, nx, nx, grid=grid, threads=th)
         return time.perf_counter() - t0
 
-    n = 16384
-    t = run(n)
-    while t < 1.0 and n < cfg.n_rows:
-        n = min(cfg.n_rows, n * 4)
-        t = run(n)
-    n = int(min(cfg.n_rows, max(n, n * target_s / max(t, 1e-6))))
-    t = run(n)
-    vis = n * cfg.n_chan * cfg.n_pol
-    return {"value": vis / t, "unit": "vis/s", "cores": threads, "kind": kind,
-            "sample": f"{n} of {cfg.n_rows} rows ({vis} vis, time-major block from the middle of "
-                      f"the observation) gridded by the reference's wstack() "
-                      f"(KAISERBESSEL, support {cfg.support}) into the full {nx}^2 x "
-                      f"{cfg.planes}-plane FP64 grid with {threads} OpenMP threads; "
-                      f"{t:.2f} s", "cpu": _cpu_model()}
+    def sample(th, target):
+        n = 16384
+        t = run(n, th)
+        while t < 0.5 and n < cfg.n_rows:
+            n = min(cfg.n_rows, n * 4)
+            t = run(n, th)
+        n = int(min(cfg.n_rows, max(n, n * target / max(t, 1e-6))))
+        return n, run(n, th)
+
+    n, t = sample(threads, target_s)
+    n1, t1 = sample(1, target_s / 2)
+    FP = cfg.n_chan * cfg.n_pol
+    rate, rate1 = n * FP / t, n1 * FP / t1
+    # FFT and phase on k planes of the gridded sample, scaled to all planes.
+    k = min(cfg.planes, 2)
+    plane = 2 * nx * nx
+    sub = np.ascontiguousarray(grid[:k * plane])
+    t0 = time.perf_counter()
+    gs = O.fft_planes(k, nx, nx, sub, threads=threads)
+    t_fft = (time.perf_counter() - t0) * cfg.planes / k
+    res, wmin, wmax = host_grid_fn()
+    t0 = time.perf_counter()
+    if kind == "reference":
+        O.ref_phase_correction(gs, nx, nx, k, res, wmin, wmax, threads=threads)
+    else:
+        O.phase_correction(gs, nx, nx, k, res, wmin, wmax, threads=threads)
+    t_phase = (time.perf_counter() - t0) * cfg.planes / k
+    del grid, sub, gs
+    t_grid_all = cfg.n_vis / rate
+    return {"value": rate, "unit": "vis/s", "cores": threads, "kind": kind,
+            "sample": f"{n} of {cfg.n_rows} rows ({n * FP} vis, time-major block from the middle "
+                      f"of the observation) gridded by the reference's wstack() (KAISERBESSEL, "
+                      f"support {cfg.support}) into the full {nx}^2 x {cfg.planes}-plane FP64 "
+                      f"grid with {threads} OpenMP threads in {t:.2f} s; {n1} rows with 1 thread "
+                      f"in {t1:.2f} s; FFT and phase timed on {k} of {cfg.planes} planes and "
+                      f"scaled",
+            "one_thread_vis_per_s": rate1,
+            "image_s": {"grid": t_grid_all, "fft": t_fft, "phase": t_phase,
+                        "total": t_grid_all + t_fft + t_phase,
+                        "fft_kind": "oracle transform (FFTW absent), all threads",
+                        "phase_kind": "reference phase_correction()" if kind == "reference"
+                        else "oracle phase_correction"},
+            "threads_used": threads, "affinity_cpus": aff, "nproc": nproc,
+            "cpu": _cpu_model()}
 
 
 def dropin_rate(ds, cfg, torch):
@@ -177,7 +236,7 @@ def dropin_rate(ds, cfg, torch):
     grid = np.zeros(2 * cfg.planes * nx * nx, dtype=np.float64)
     args = (cfg.planes, cfg.n_rows, cfg.n_chan, cfg.n_pol, h.uu, h.vv, h.ww, h.vis_real,
             h.vis_img, h.weight, 1.0 / nx, 1.0 / cfg.planes, cfg.support, nx, nx, grid)
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    threads = host_cores()[0]
     imaging.set_kernel(cfg.kernel)
     imaging.wstack(*args, num_threads=threads)          # warm-up: plan + pinned buffers
     t0 = time.perf_counter()
@@ -275,6 +334,96 @@ def binms_rates(ds, cfg, torch, path):
             "read_device_GBs": nbytes / td / 1e9, "device_read_matches": ok}
 
 
+def batch_leg(torch, steps=3):
+    """The batched workload: 96 independent imaging jobs (grids 256-2048,
+    4-16 w-planes, 5e4-8e6 visibilities, Gaussian support 7 or Kaiser-Bessel
+    support 9), each job's chain captured into one HIP graph per precision,
+    against the same jobs imaged one after another in FP64.  Every job's
+    image is compared with its sequential FP64 image."""
+    from rick_amd import FP32, FP64, batch, synth
+    cfgs = batch.job_configs(96)
+    datasets = [synth.make_dataset(c, device="cuda:0") for c in cfgs]
+    total_vis = sum(c.n_vis for c in cfgs)
+    seq = batch.make_jobs(cfgs, FP64, datasets=datasets)
+    for j in seq:
+        j.run()
+
+    def run_seq():
+        for j in seq:
+            j.run()
+    t_seq = timed(run_seq, steps, torch) * 1e-3
+    ref = [j.re.cpu().numpy() + 1j * j.im.cpu().numpy() for j in seq]
+    del seq
+    torch.cuda.empty_cache()
+    out = {"jobs": len(cfgs), "visibilities": total_vis,
+           "grids": sorted({c.grid for c in cfgs}), "seq_fp64_ms": t_seq * 1e3,
+           "seq_fp64_images_per_s": len(cfgs) / t_seq}
+    for name, prec in (("fp64", FP64), ("fp32", FP32)):
+        b = batch.Batch(batch.make_jobs(cfgs, prec, datasets=datasets))
+        b.capture()
+        t_graph = timed(b.replay, steps, torch) * 1e-3
+        t_eager = timed(b.run_eager, steps, torch) * 1e-3
+        errs = [float(np.linalg.norm(a - r) / np.linalg.norm(r)) for a, r in zip(b.images(), ref)]
+        out[name] = {"graph_ms": t_graph * 1e3, "eager_ms": t_eager * 1e3,
+                     "images_per_s": len(cfgs) / t_graph, "vis_per_s": total_vis / t_graph,
+                     "max_rel_l2_vs_seq_fp64": max(errs)}
+        del b
+        torch.cuda.empty_cache()
+    out["fp32_graph_speedup_vs_seq_fp64"] = out["seq_fp64_ms"] / out["fp32"]["graph_ms"]
+    del datasets
+    torch.cuda.empty_cache()
+    return out
+
+
+def serve_leg(torch, seconds, rate=400.0, slots=8, budget_gb=256.0):
+    """The serving workload: the batched job mix plus 10 % LOFAR-like
+    4096^2 x 32-plane jobs of 20M visibilities, Poisson arrivals, admission by
+    an HBM budget.  Reported: latency percentiles at `rate` requests/s for
+    the FP32 continuous-batching service and for sequential FP64; the
+    saturated rate of each (every request of a burst queued at once); and the
+    highest offered rate of a doubling sweep whose p99 stays within 4x the
+    p99 at `rate` while keeping up with the arrivals."""
+    from rick_amd import FP32, FP64, serve, synth
+    cfgs, weights = serve.workload()
+    datasets = [synth.make_dataset(c, device="cuda:0") for c in cfgs]
+    budget = int(budget_gb * 1e9)
+    ex32 = serve.GpuExecutor(datasets, FP32, slots=slots)
+    ex64 = serve.GpuExecutor(datasets, FP64, slots=1)
+    for ex, sl in ((ex32, slots), (ex64, 1)):      # plan pools: creation allocates
+        serve.Server(ex, budget, sl).run([serve.Request(i, i, 0.0) for i in range(len(cfgs))])
+
+    def run(ex, sl, r, dur, seed=7):
+        reqs = serve.poisson_requests(len(cfgs), weights, r, dur, seed)
+        return serve.latency_stats(serve.Server(ex, budget, sl).run(reqs)), len(reqs)
+
+    out = {"jobs": len(cfgs), "lofar_jobs": sum(c.grid == 4096 for c in cfgs),
+           "slots": slots, "budget_gb": budget_gb, "rate_per_s": rate, "seconds": seconds}
+    out["fp32_continuous"], n = run(ex32, slots, rate, seconds)
+    out["fp64_sequential"], _ = run(ex64, 1, rate, seconds)
+    out["requests"] = n
+    for name, ex, sl in (("fp32_continuous", ex32, slots), ("fp64_sequential", ex64, 1)):
+        burst = [serve.Request(i, r.key, 0.0) for i, r in
+                 enumerate(serve.poisson_requests(len(cfgs), weights, 200.0, 2.0, seed=11))]
+        st = serve.latency_stats(serve.Server(ex, budget, sl).run(burst))
+        out[name]["saturated_images_per_s"] = st["images_per_s"]
+    p99_ref = out["fp32_continuous"]["p99_ms"]
+    best, r = None, rate
+    sweep = []
+    while r <= 64 * rate:
+        st, n = run(ex32, slots, r, max(2.0, seconds / 4), seed=int(r))
+        ok = st["p99_ms"] <= 4 * p99_ref and st["images_per_s"] >= 0.9 * r
+        sweep.append({"rate_per_s": r, "p99_ms": st["p99_ms"], "images_per_s": st["images_per_s"],
+                      "bounded": ok})
+        if not ok:
+            break
+        best, r = r, 2 * r
+    out["fp32_rate_sweep"] = sweep
+    out["fp32_max_bounded_rate_per_s"] = best
+    del ex32, ex64, datasets
+    torch.cuda.empty_cache()
+    return out
+
+
 def copy_bandwidth(torch, gib=1, reps=5):
     """Device-to-device copy of ``gib`` GiB (read + write counted), timed with
     events: the practical HBM ceiling beside the published 8 TB/s."""
@@ -331,25 +480,28 @@ def main():
     for name, prec in (("fp64", FP64), ("fp32", FP32)):
         if args.only and args.only != name:
             continue
-        tile, rpt = (0, 0), args.rows_per_task
+        tile, rpt, unroll = (0, 0), args.rows_per_task, 0
         if args.tile:
             tile = tuple(int(x) for x in args.tile.lower().split("x"))
         elif args.tune:
             best = autotune.tune_grid(ds, prec, log=log)["best"]
-            tile, rpt = tuple(best["tile"]), best["rows_per_task"]
+            tile, rpt, unroll = tuple(best["tile"]), best["rows_per_task"], best["unroll"]
         else:
             tuned = autotune.lookup(cfg.grid, cfg.planes, cfg.support, prec, cfg.n_rows)
             if tuned:
-                tile, rpt = tuned
-        log(f"{name}: tile {tile} rows/task {rpt}; warmup {args.warmup}, steps {args.steps}")
+                tile, rpt, unroll = tuned
+        log(f"{name}: tile {tile} rows/task {rpt} unroll {unroll or 'default'}; warmup "
+            f"{args.warmup}, steps {args.steps}")
         plan, gms, ims, stages, istages, stats, img = run_path(ds, ins, prec, args.steps,
-                                                               args.warmup, torch, tile, rpt)
+                                                               args.warmup, torch, tile, rpt,
+                                                               unroll)
         res[name] = {"grid_ms": gms, "image_ms": ims, "grid_vis_per_s": vis / (gms * 1e-3),
                      "stages": stages, "image_stages": istages, "grid_stats": stats,
                      "img": img,
                      "grid_bytes": plan.grid_bytes,
                      "tile": [plan.desc.tile_u or 64, plan.desc.tile_v or 64],
-                     "rows_per_task": plan.desc.rows_per_task or 4096}
+                     "rows_per_task": plan.desc.rows_per_task or 4096,
+                     "footprint_unroll": plan.desc.footprint_unroll or None}
         log(f"{name}: grid {gms:.3f} ms ({vis / gms / 1e6:.3f} Gvis/s), image {ims:.3f} ms")
         del plan
         torch.cuda.empty_cache()
@@ -428,6 +580,17 @@ def main():
     if args.binms:
         log(f"binMS write and read-back under {args.binms}")
         out["binms"] = binms_rates(ds, cfg, torch, args.binms)
+    if not args.no_batch and not args.only:
+        log("batched imaging: 96 jobs, one HIP graph per precision")
+        out["batch"] = batch_leg(torch)
+        log(f"batch: fp32 graph {out['batch']['fp32']['graph_ms']:.1f} ms, fp64 graph "
+            f"{out['batch']['fp64']['graph_ms']:.1f} ms, sequential fp64 "
+            f"{out['batch']['seq_fp64_ms']:.1f} ms")
+    if not args.no_serve and not args.only:
+        log(f"serving: {args.serve_seconds:.0f} s of Poisson arrivals per run")
+        out["serve"] = serve_leg(torch, args.serve_seconds)
+        log(f"serve: fp32 p99 {out['serve']['fp32_continuous']['p99_ms']:.2f} ms, fp64 "
+            f"sequential p99 {out['serve']['fp64_sequential']['p99_ms']:.2f} ms")
     if not args.no_dropin and not args.only:
         log("drop-in wstack() on host arrays")
         out["dropin_pcie_inclusive"] = dropin_rate(ds, cfg, torch)
@@ -442,8 +605,16 @@ def main():
                     np.ascontiguousarray(sub.ww), np.ascontiguousarray(sub.vis_real),
                     np.ascontiguousarray(sub.vis_img), np.ascontiguousarray(sub.weight))
 
-        out["cpu_baseline"] = cpu_baseline(host_rows, cfg, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(host_rows, cfg, args.cpu_seconds,
+                                           lambda: (ds.resolution, ds.wmin, ds.wmax))
         out["gpu_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+        if "fp64" in res:
+            cpu_img = out["cpu_baseline"]["image_s"]["total"]
+            out["image_time_vs_cpu"] = {
+                "cpu_image_s": cpu_img,
+                "gpu_fp64_image_s": res["fp64"]["image_ms"] * 1e-3,
+                "gpu_fp32_image_s": res["fp32"]["image_ms"] * 1e-3 if "fp32" in res else None,
+                "fp64_speedup": cpu_img / (res["fp64"]["image_ms"] * 1e-3)}
     print(json.dumps(out))
 
 

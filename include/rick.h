@@ -80,8 +80,11 @@ void phase_correction(double *gridss, double *image_real, double *image_imag,
 /* Comm-free replacement of the FFT stage: fftw_data() (fourier_transform.c:
  * 13-93, FFTW_BACKWARD, n0 = Ny, n1 = Nx, scaled by 1/(Nx*Ny)) and cuda_fft()
  * (cuda_fft.cu:62-199) for one rank.  Reads grid, writes gridss (host arrays,
- * same layout; may alias).  Power-of-two axes up to 8192.  Single rank:
- * xaxis == grid_size_x and yaxis == grid_size_y. */
+ * same layout; may alias).  Any axis length >= 2 (power-of-two axes in
+ * [16, 8192] take the Stockham engine; other lengths mixed radix 2..16, 3,
+ * 5, 7, 11, 13, four-step for long or strided axes, Bluestein for larger
+ * prime factors).  Single rank: xaxis == grid_size_x and yaxis ==
+ * grid_size_y. */
 void rick_fft_planes(int num_w_planes, int grid_size_x, int grid_size_y,
                      int xaxis, int yaxis, const double *grid, double *gridss,
                      int rank);
@@ -126,6 +129,10 @@ typedef struct {
      * reference does, local index against the totals (phase_correction.cu:
      * 204-209).  0 = the plan's own grid size. */
     int xaxistot, yaxistot;
+    /* Gridder: length of the unrolled footprint loops, one of 7, 9, 11, 15
+     * and >= w_support (a tuned choice, rick_amd.autotune); 0 = the smallest
+     * that fits. */
+    int footprint_unroll;
 } rick_plan_desc;
 
 int rick_plan_create(rick_plan *plan, const rick_plan_desc *desc);

@@ -30,7 +30,7 @@ class PlanDesc(ctypes.Structure):
         ("precision", _c_int), ("max_points", _c_uint), ("resolution", _c_double),
         ("wmin", _c_double), ("wmax", _c_double), ("phase_on", _c_int), ("tile_u", _c_int),
         ("tile_v", _c_int), ("rows_per_task", _c_int), ("device", _c_int),
-        ("xaxistot", _c_int), ("yaxistot", _c_int),
+        ("xaxistot", _c_int), ("yaxistot", _c_int), ("footprint_unroll", _c_int),
     ]
 
 

@@ -2,9 +2,12 @@
 
 The gridder's tile shape (TU x TV cells per LDS tile, i.e. per bucket) and the
 records per task (rows_per_task) trade occupancy (LDS per workgroup), record
-duplication across tile borders and load balance on hot tiles.  This module
-times the candidates on the actual workload and caches the fastest per
-workload signature in a JSON file (rick_amd/tuning.json by default).
+duplication across tile borders and load balance on hot tiles; the footprint
+unroll (the compile-time length of the tile kernel's footprint loops, any
+instantiated length >= the support) trades registers against dead
+iterations.  This module times the candidates on the actual workload and
+caches the fastest per workload signature in a JSON file
+(rick_amd/tuning.json by default).
 
     python -m rick_amd.autotune --config lofar --precision fp32
     python -m rick_amd.autotune --batch 96 --precision fp32   # the batched job mix
@@ -19,6 +22,7 @@ from pathlib import Path
 
 TILES = [(64, 64), (64, 32), (32, 32), (128, 32), (32, 64)]
 ROWS_PER_TASK = [1024, 4096, 16384]
+UNROLLS = [7, 9, 11, 15]     # instantiated footprint unrolls (rick_plan_desc.footprint_unroll)
 CACHE = Path(__file__).resolve().parent / "tuning.json"
 
 
@@ -35,46 +39,61 @@ def load(path: Path = CACHE) -> dict:
 
 
 def lookup(grid, planes, support, precision, rows, path: Path = CACHE):
-    """(tile, rows_per_task) tuned for this shape, or None."""
+    """(tile, rows_per_task, unroll) tuned for this shape, or None; entries
+    tuned before the unroll was a candidate give unroll 0 (the smallest)."""
     e = load(path).get(signature(grid, planes, support, precision, rows))
     if not e:
         return None
-    return tuple(e["tile"]), int(e["rows_per_task"])
+    return tuple(e["tile"]), int(e["rows_per_task"]), int(e.get("unroll", 0))
 
 
-def tune_grid(ds, precision, tiles=None, rpts=None, steps=3, log=print):
+def unrolls_for(support: int) -> list:
+    return [u for u in UNROLLS if u >= support]
+
+
+def tune_grid(ds, precision, tiles=None, rpts=None, steps=3, log=print, unrolls=None):
     """Time rick_plan_grid for each candidate on dataset ``ds`` (inputs on the
-    GPU); returns {"best": {...}, "results": [...]}."""
+    GPU): tile shapes x rows per task at the smallest unroll, then the other
+    unrolls at the best shape.  Returns {"best": {...}, "results": [...]}."""
     import torch
 
     from .imaging import plan_for
     cfg = ds.cfg
     ins = [ds.uu, ds.vv, ds.ww, ds.vis_real, ds.vis_img, ds.weight]
     results = []
+
+    def run(tile, rpt, unroll):
+        plan = plan_for(ds, precision=precision, tile=tile, rows_per_task=rpt, unroll=unroll)
+        grid = plan.new_grid()
+        plan.grid(*ins, cfg.n_chan, cfg.n_pol, out=grid)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            plan.grid(*ins, cfg.n_chan, cfg.n_pol, out=grid)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        results.append({"tile": list(tile), "rows_per_task": rpt, "unroll": unroll, "ms": ms})
+        log(f"[tune] tile {tile[0]}x{tile[1]} rows/task {rpt} unroll {unroll}: {ms:.3f} ms")
+        del grid, plan
+        torch.cuda.empty_cache()
+
+    cand = unrolls or unrolls_for(cfg.support)
     for tile in tiles or TILES:
         if tile[0] < cfg.support or tile[1] < cfg.support:
             continue
         for rpt in rpts or ROWS_PER_TASK:
-            plan = plan_for(ds, precision=precision, tile=tile, rows_per_task=rpt)
-            grid = plan.new_grid()
-            plan.grid(*ins, cfg.n_chan, cfg.n_pol, out=grid)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                plan.grid(*ins, cfg.n_chan, cfg.n_pol, out=grid)
-            torch.cuda.synchronize()
-            ms = (time.perf_counter() - t0) * 1e3 / steps
-            results.append({"tile": list(tile), "rows_per_task": rpt, "ms": ms})
-            log(f"[tune] tile {tile[0]}x{tile[1]} rows/task {rpt}: {ms:.3f} ms")
-            del grid, plan
-            torch.cuda.empty_cache()
+            run(tile, rpt, cand[0])
+    shape = min(results, key=lambda r: r["ms"])
+    for u in cand[1:]:
+        run(tuple(shape["tile"]), shape["rows_per_task"], u)
     best = min(results, key=lambda r: r["ms"])
     return {"best": best, "results": results}
 
 
 def store(key: str, best: dict, path: Path = CACHE) -> None:
     data = load(path)
-    data[key] = {"tile": best["tile"], "rows_per_task": best["rows_per_task"], "ms": best["ms"]}
+    data[key] = {"tile": best["tile"], "rows_per_task": best["rows_per_task"],
+                 "unroll": best.get("unroll", 0), "ms": best["ms"]}
     tmp = Path(str(path) + ".tmp")
     tmp.write_text(json.dumps(data, indent=1, sort_keys=True))
     os.replace(tmp, path)
@@ -97,7 +116,7 @@ def tune_batch(n_jobs, precision, out, steps=2, log=print, seed=3):
         store(key, r["best"], Path(out))
         seen[key] = r["best"]
         log(f"[tune] {key}: tile {r['best']['tile']} rows/task {r['best']['rows_per_task']} "
-            f"{r['best']['ms']:.3f} ms")
+            f"unroll {r['best']['unroll']} {r['best']['ms']:.3f} ms")
         del ds
         torch.cuda.empty_cache()
     return seen

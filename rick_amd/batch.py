@@ -80,12 +80,12 @@ def make_jobs(cfgs, precision, device="cuda:0", tuned=True, datasets=None):
     jobs = []
     for i, cfg in enumerate(cfgs):
         ds = datasets[i] if datasets is not None else synth.make_dataset(cfg, device=device)
-        tile, rpt = (0, 0), 0
+        tile, rpt, unroll = (0, 0), 0, 0
         if tuned:
             t = autotune.lookup(cfg.grid, cfg.planes, cfg.support, precision, cfg.n_rows)
             if t:
-                tile, rpt = t
-        plan = plan_for(ds, precision=precision, tile=tile, rows_per_task=rpt)
+                tile, rpt, unroll = t
+        plan = plan_for(ds, precision=precision, tile=tile, rows_per_task=rpt, unroll=unroll)
         npix = cfg.grid * cfg.grid
         jobs.append(Job(ds, plan, plan.new_grid(),
                         torch.empty(npix, dtype=torch.float64, device=device),

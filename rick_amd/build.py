@@ -27,7 +27,8 @@ HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-value",
              "-Wno-unused-result"]
-HIP_SOURCES = ["grid.hip", "fft.hip", "phase.hip", "prep.hip", "api.cpp", "io.cpp"]
+HIP_SOURCES = ["grid.hip", "fft.hip", "fft_gen.hip", "phase.hip", "prep.hip", "api.cpp",
+               "io.cpp"]
 CXX_SOURCES = ["tables.cpp"]
 
 

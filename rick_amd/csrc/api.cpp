@@ -87,7 +87,9 @@ struct rick_plan_s {
     size_t dbytes = 0;
     double *table_d = nullptr;
     void *tmp_plane = nullptr;
-    bool fft_ok = false;
+    bool fft_ok = false;    // power-of-two axes in [16, 8192]: the Stockham engine of fft.hip
+    bool gen_ok = false;    // any other axis lengths: the mixed-radix / Bluestein engine
+    GenFft gen{};
     bool grid_ok = false;
     Timing timing;
 };
@@ -190,6 +192,18 @@ int rick_plan_create(rick_plan *out, const rick_plan_desc *desc)
     g.ntable = kTableOversample * d.w_support;
     // ~64-128 KB of grid per zero task measured best (LOFAR-like grid).
     g.zrun = d.precision == RICK_FP64 ? 2 : 4;
+    // Unrolled footprint loops of the tile kernel: the instantiated lengths.
+    {
+        const int unrolls[] = {7, 9, 11, 15};
+        g.smax = 0;
+        for (int u : unrolls)
+            if (u >= d.w_support && (d.footprint_unroll == 0 ? g.smax == 0 : u == d.footprint_unroll))
+                g.smax = u;
+        if (g.smax == 0)
+            return set_error(RICK_ERR_INVALID,
+                             "footprint_unroll %d: must be 0 or one of 7, 9, 11, 15 and >= "
+                             "w_support %d", d.footprint_unroll, d.w_support);
+    }
     if ((uint64_t)d.max_points * 4ull >= (1ull << 32))
         return set_error(RICK_ERR_INVALID, "max_points %u too large (limit 2^30)", d.max_points);
 
@@ -201,6 +215,7 @@ int rick_plan_create(rick_plan *out, const rick_plan_desc *desc)
     }
     p->fft_ok = is_pow2(g.nx) && is_pow2(g.ny) && g.nx >= 16 && g.ny >= 16 && g.nx <= 8192 &&
                 g.ny <= 8192;
+    if (!p->fft_ok) p->gen_ok = gen_fft_prepare(g.nx, g.ny, d.precision, &p->gen);
 
     // Device memory: [table | twiddles x | twiddles y | l table | m table |
     //                 tmp plane | gridding workspace].
@@ -214,6 +229,10 @@ int rick_plan_create(rick_plan *out, const rick_plan_desc *desc)
     const size_t o_m = off; off += al(sizeof(double) * (size_t)g.ny);
     const size_t o_tmp = off;
     if (p->fft_ok && fft_needs_tmp(d.precision, g.ny)) off += al(csz * (size_t)g.nx * g.ny);
+    const size_t o_gen_tab = off;
+    if (p->gen_ok) off += al(p->gen.table_bytes);
+    const size_t o_gen_work = off;
+    if (p->gen_ok) off += al(p->gen.work_bytes);
     const size_t o_work = off;
     if (p->grid_ok) off += grid_work_bytes(g, d.max_points, d.precision);
 
@@ -228,8 +247,16 @@ int rick_plan_create(rick_plan *out, const rick_plan_desc *desc)
     p->dbytes = off;
     char *base = static_cast<char *>(p->dmem);
     p->table_d = reinterpret_cast<double *>(base + o_table);
-    if (o_tmp != o_work) p->tmp_plane = base + o_tmp;
+    if (p->fft_ok && fft_needs_tmp(d.precision, g.ny)) p->tmp_plane = base + o_tmp;
     if (p->grid_ok) grid_work_carve(g, d.max_points, d.precision, base + o_work, &p->work);
+    // The FP32 split-bucket accumulators and tickets start at zero (the
+    // gridder leaves them zero after every call).
+    if (p->grid_ok) e = hipMemset(base + o_work, 0, grid_work_bytes(g, d.max_points, d.precision));
+    if (e != hipSuccess) {
+        hipFree(p->dmem);
+        hipSetDevice(prev);
+        return set_error(RICK_ERR_DEVICE, "workspace clear failed: %s", hipGetErrorString(e));
+    }
 
     // Host tables -> device (synchronous; plan creation is not on a hot path).
     std::vector<double> h(kMaxTable, 0.0);
@@ -258,6 +285,8 @@ int rick_plan_create(rick_plan *out, const rick_plan_desc *desc)
         e = hipMemcpy(base + o_l, lt.data(), sizeof(double) * g.nx, hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(base + o_m, mt.data(), sizeof(double) * g.ny, hipMemcpyHostToDevice);
+    if (e == hipSuccess && p->gen_ok)
+        e = gen_fft_upload(&p->gen, d.precision, base + o_gen_tab, base + o_gen_work);
     hipSetDevice(prev);
     if (e != hipSuccess) {
         hipFree(p->dmem);
@@ -314,16 +343,19 @@ int rick_plan_fft(rick_plan p, const void *in, void *out, void *stream)
 {
     clear_error();
     if (!p) return set_error(RICK_ERR_INVALID, "null plan");
-    if (!p->fft_ok)
-        return set_error(RICK_ERR_UNSUPPORTED,
-                         "FFT needs power-of-two axes in [16, 8192], got %d x %d", p->geom.nx,
-                         p->geom.ny);
+    if (!p->fft_ok && !p->gen_ok)
+        return set_error(RICK_ERR_UNSUPPORTED, "no FFT plan for %d x %d (axes must be >= 2)",
+                         p->geom.nx, p->geom.ny);
     if (!in || !out) return set_error(RICK_ERR_INVALID, "null buffer");
     RICK_HIP(hipSetDevice(p->d.device));
     // fourier_transform.c:24
     const double norm = 1.0 / (double)((double)p->geom.nx * (double)p->geom.ny);
-    RICK_HIP(launch_fft(p->fft, p->d.precision, in, out, p->tmp_plane, norm,
-                        static_cast<hipStream_t>(stream), &p->timing));
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (p->fft_ok)
+        RICK_HIP(launch_fft(p->fft, p->d.precision, in, out, p->tmp_plane, norm, s, &p->timing));
+    else
+        RICK_HIP(launch_fft_gen(p->gen, p->d.precision, p->geom.planes, p->geom.nx, p->geom.ny,
+                                in, out, norm, s, &p->timing));
     return RICK_OK;
 }
 
@@ -393,10 +425,9 @@ int rick_plan_image(rick_plan p, unsigned int n, unsigned int F, unsigned int P,
     clear_error();
     if (!p) return set_error(RICK_ERR_INVALID, "null plan");
     if (!p->grid_ok) return set_error(RICK_ERR_INVALID, "plan has no gridding capacity");
-    if (!p->fft_ok)
-        return set_error(RICK_ERR_UNSUPPORTED,
-                         "FFT needs power-of-two axes in [16, 8192], got %d x %d", p->geom.nx,
-                         p->geom.ny);
+    if (!p->fft_ok && !p->gen_ok)
+        return set_error(RICK_ERR_UNSUPPORTED, "no FFT plan for %d x %d (axes must be >= 2)",
+                         p->geom.nx, p->geom.ny);
     if (n > p->d.max_points)
         return set_error(RICK_ERR_INVALID, "num_points %u exceeds plan capacity %u", n,
                          p->d.max_points);
@@ -409,6 +440,17 @@ int rick_plan_image(rick_plan p, unsigned int n, unsigned int F, unsigned int P,
         return set_error(RICK_ERR_INVALID, "wmax == wmin: the w-plane spacing is undefined");
     RICK_HIP(hipSetDevice(p->d.device));
     const hipStream_t s = static_cast<hipStream_t>(stream);
+    const double norm = 1.0 / (double)((double)p->geom.nx * (double)p->geom.ny);
+    if (!p->fft_ok) {
+        // Other axis lengths: dense grid, the mixed-radix / Bluestein FFT in
+        // place, then the phase correction and plane sum.
+        RICK_HIP(launch_grid(p->geom, p->work, p->d.precision, p->table_d, n, F, P, uu, vv, ww,
+                             vr, vi, wt, grid, 0, s, &p->timing, true));
+        RICK_HIP(launch_fft_gen(p->gen, p->d.precision, p->geom.planes, p->geom.nx, p->geom.ny,
+                                grid, grid, norm, s, &p->timing));
+        RICK_HIP(launch_phase(p->phase, p->d.precision, grid, re, im, 0, s, &p->timing));
+        return RICK_OK;
+    }
     // Sparse grid: tiles of empty buckets are left unwritten; the row FFT
     // reads only occupied tiles (bucket counts) and treats the rest as zero.
     // (A row of more than 256 tiles exceeds the row FFT's occupancy table;
@@ -416,7 +458,6 @@ int rick_plan_image(rick_plan p, unsigned int n, unsigned int F, unsigned int P,
     const bool sparse = p->geom.nx / p->geom.tu <= 256;
     RICK_HIP(launch_grid(p->geom, p->work, p->d.precision, p->table_d, n, F, P, uu, vv, ww, vr,
                          vi, wt, grid, 0, s, &p->timing, !sparse));
-    const double norm = 1.0 / (double)((double)p->geom.nx * (double)p->geom.ny);
     RICK_HIP(launch_image_chain(p->fft, p->d.precision, grid, norm, s, &p->timing,
                                 sparse ? &p->geom : nullptr, sparse ? p->work.counts : nullptr,
                                 sparse ? p->work.band : nullptr, p->phase, re, im));

@@ -23,6 +23,7 @@
 // Twiddle tables hold exp(+2*pi*i*m/N) for the axis length N; an L-point
 // sub-transform uses them with stride N/L.  Per butterfly only W^k, W^2k,
 // W^4k, W^8k are loaded; the other powers are products of those.
+#include "fft_kernels.h"
 #include "rick_internal.h"
 #include "rick_math.h"
 
@@ -31,113 +32,57 @@
 namespace rick {
 namespace {
 
+using namespace fftk;
+
 constexpr int kE = 16;  // complex values per thread
 
+// The fused column/phase pass: complex values per thread for FP64 and FP32
+// plane data, and whether FP64 prefetches the next plane into registers.
+#ifndef RICK_PHASE_E_F64
+#define RICK_PHASE_E_F64 8
+#endif
+#ifndef RICK_PHASE_E_F32
+#define RICK_PHASE_E_F32 8
+#endif
+#ifndef RICK_PHASE_PREFETCH_F64
+#define RICK_PHASE_PREFETCH_F64 0
+#endif
+// FP64: the w-phase rotation of a pixel advances plane to plane by a fixed
+// step (exp(i 2 pi dw (n-1))), one complex multiply per plane instead of a
+// sine and cosine.
+#ifndef RICK_PHASE_RECUR_F64
+#define RICK_PHASE_RECUR_F64 1
+#endif
+constexpr bool kPhasePrefetchF64 = RICK_PHASE_PREFETCH_F64 != 0;
+constexpr bool kPhaseRecurF64 = RICK_PHASE_RECUR_F64 != 0;
 template <typename T>
-struct C2 {
-    T x, y;
-};
-
-template <typename T>
-__device__ __forceinline__ C2<T> cadd(C2<T> a, C2<T> b) { return {a.x + b.x, a.y + b.y}; }
-template <typename T>
-__device__ __forceinline__ C2<T> csub(C2<T> a, C2<T> b) { return {a.x - b.x, a.y - b.y}; }
-template <typename T>
-__device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b)
+constexpr int phase_values(int L)
 {
-    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
-}
-// Multiplication by +i (backward transforms use exp(+2 pi i ...)).
-template <typename T>
-__device__ __forceinline__ C2<T> mul_i(C2<T> a) { return {-a.y, a.x}; }
-
-// exp(+2*pi*i*m/16) with m known at compile time.
-template <typename T, int M>
-__device__ __forceinline__ C2<T> rot16(C2<T> a)
-{
-    constexpr int m = M & 15;
-    constexpr double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173,
-                     h = 0.70710678118654752440;
-    if constexpr (m == 0) return a;
-    else if constexpr (m == 4) return mul_i(a);
-    else if constexpr (m == 8) return {-a.x, -a.y};
-    else if constexpr (m == 12) return {a.y, -a.x};
-    else {
-        constexpr double tr[16] = {1, c1, h, s1, 0, -s1, -h, -c1, -1, -c1, -h, -s1, 0, s1, h, c1};
-        constexpr double re = tr[m], im = tr[(m + 12) & 15];
-        return cmul(a, C2<T>{(T)re, (T)im});
-    }
+    // Lines shorter than the per-thread count keep 16 (one butterfly per line).
+    return L < 16 ? kE : (sizeof(T) == 8 ? RICK_PHASE_E_F64 : RICK_PHASE_E_F32);
 }
 
-template <typename T>
-__device__ __forceinline__ void dft4(C2<T> &x0, C2<T> &x1, C2<T> &x2, C2<T> &x3)
-{
-    const C2<T> a = cadd(x0, x2), b = csub(x0, x2), c = cadd(x1, x3), d = mul_i(csub(x1, x3));
-    x0 = cadd(a, c);
-    x2 = csub(a, c);
-    x1 = cadd(b, d);
-    x3 = csub(b, d);
-}
-
-// In-register DFT with sign +1: X[k] = sum_n x[n] exp(+2 pi i n k / R).
-template <typename T, int R>
-__device__ __forceinline__ void dftR(C2<T> *v)
-{
-    if constexpr (R == 2) {
-        const C2<T> a = v[0], b = v[1];
-        v[0] = cadd(a, b);
-        v[1] = csub(a, b);
-    } else if constexpr (R == 4) {
-        dft4(v[0], v[1], v[2], v[3]);
-    } else if constexpr (R == 8) {
-        // n = 2*n1 + n2, k = k1 + 4*k2.
-        C2<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
-        C2<T> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
-        dft4(e0, e1, e2, e3);
-        dft4(o0, o1, o2, o3);
-        o1 = rot16<T, 2>(o1);
-        o2 = rot16<T, 4>(o2);
-        o3 = rot16<T, 6>(o3);
-        v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
-        v[1] = cadd(e1, o1); v[5] = csub(e1, o1);
-        v[2] = cadd(e2, o2); v[6] = csub(e2, o2);
-        v[3] = cadd(e3, o3); v[7] = csub(e3, o3);
-    } else {
-        // n = 4*n1 + n2, k = k1 + 4*k2.
-        C2<T> y[4][4];
-#pragma unroll
-        for (int n2 = 0; n2 < 4; ++n2) {
-            y[n2][0] = v[n2];
-            y[n2][1] = v[4 + n2];
-            y[n2][2] = v[8 + n2];
-            y[n2][3] = v[12 + n2];
-            dft4(y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
-        }
-        y[1][1] = rot16<T, 1>(y[1][1]); y[1][2] = rot16<T, 2>(y[1][2]); y[1][3] = rot16<T, 3>(y[1][3]);
-        y[2][1] = rot16<T, 2>(y[2][1]); y[2][2] = rot16<T, 4>(y[2][2]); y[2][3] = rot16<T, 6>(y[2][3]);
-        y[3][1] = rot16<T, 3>(y[3][1]); y[3][2] = rot16<T, 6>(y[3][2]); y[3][3] = rot16<T, 9>(y[3][3]);
-#pragma unroll
-        for (int k1 = 0; k1 < 4; ++k1) {
-            dft4(y[0][k1], y[1][k1], y[2][k1], y[3][k1]);
-            v[k1] = y[0][k1];
-            v[k1 + 4] = y[1][k1];
-            v[k1 + 8] = y[2][k1];
-            v[k1 + 12] = y[3][k1];
-        }
-    }
-}
-
-// Compile-time pass plan of an L-point transform.
-constexpr int n16(int L)
+// Compile-time pass plan of an L-point transform with E values per thread:
+// radix-E passes, then one pass of the remaining power of two.
+constexpr int nE(int L, int E)
 {
     int n = 0;
-    while (L >= 16 && L % 16 == 0) { L /= 16; ++n; }
+    while (L >= E && L % E == 0) { L /= E; ++n; }
     return n;
 }
-constexpr int pow16(int n) { return n == 0 ? 1 : 16 * pow16(n - 1); }
-constexpr int npasses(int L) { return n16(L) + (L / pow16(n16(L)) > 1 ? 1 : 0); }
-constexpr int radix_of(int L, int p) { return p < n16(L) ? 16 : L / pow16(n16(L)); }
-constexpr int ns_of(int L, int p) { return p == 0 ? 1 : ns_of(L, p - 1) * radix_of(L, p - 1); }
+constexpr int powE(int n, int E) { return n == 0 ? 1 : E * powE(n - 1, E); }
+constexpr int npasses(int L, int E = kE)
+{
+    return nE(L, E) + (L / powE(nE(L, E), E) > 1 ? 1 : 0);
+}
+constexpr int radix_of(int L, int p, int E = kE)
+{
+    return p < nE(L, E) ? E : L / powE(nE(L, E), E);
+}
+constexpr int ns_of(int L, int p, int E = kE)
+{
+    return p == 0 ? 1 : ns_of(L, p - 1, E) * radix_of(L, p - 1, E);
+}
 
 __host__ __device__ constexpr int lpad(int n) { return n + (n >> 4); }
 
@@ -225,12 +170,12 @@ struct GlobalSink {
 
 // First-pass input of this thread: element n of line g for butterfly slot
 // i*R + r, or zero where the line does not exist or the input is known zero.
-template <typename T, int L, int G, bool ROWS>
+template <typename T, int L, int G, bool ROWS, int E = kE>
 __device__ __forceinline__ void load_first(C2<T> *vals, int tid, const Lines &ln,
                                            const C2<T> *gin)
 {
-    constexpr int R = radix_of(L, 0), LR = L / R, BPT = kE / R;
-    constexpr int NT = G * L / kE;
+    constexpr int R = radix_of(L, 0, E), LR = L / R, BPT = E / R;
+    constexpr int NT = G * L / E;
 #pragma unroll
     for (int i = 0; i < BPT; ++i) {
         int g, j;
@@ -263,15 +208,15 @@ __device__ __forceinline__ int lds_at(int g, int n)
 // buffer holds one T per element (half the bytes) at the cost of two more
 // barriers per exchange.
 template <typename T, int L, int G, bool ROWS, int P, typename Sink, bool PRE = false,
-          bool SPLIT = false>
+          bool SPLIT = false, int E = kE>
 __device__ __forceinline__ void pass(C2<T> *vals, int tid, const Lines &ln, int ts,
                                      const C2<T> *__restrict__ tw, C2<T> *lds, const C2<T> *gin,
                                      Sink &sink)
 {
-    constexpr int NP = npasses(L);
+    constexpr int NP = npasses(L, E);
     if constexpr (P < NP) {
-        constexpr int R = radix_of(L, P), Ns = ns_of(L, P), LR = L / R, BPT = kE / R;
-        constexpr int NT = G * L / kE;
+        constexpr int R = radix_of(L, P, E), Ns = ns_of(L, P, E), LR = L / R, BPT = E / R;
+        constexpr int NT = G * L / E;
         constexpr bool from_lds = P > 0, to_lds = P + 1 < NP;
         if constexpr (!PRE) {
 #pragma unroll
@@ -318,7 +263,7 @@ __device__ __forceinline__ void pass(C2<T> *vals, int tid, const Lines &ln, int 
         if constexpr (to_lds && SPLIT) {
             // Outputs of this pass are in vals; move them to the next pass's
             // layout one component at a time.
-            constexpr int R2 = radix_of(L, P + 1), LR2 = L / R2, BPT2 = kE / R2;
+            constexpr int R2 = radix_of(L, P + 1, E), LR2 = L / R2, BPT2 = E / R2;
             T *lt = reinterpret_cast<T *>(lds);
 #pragma unroll
             for (int comp = 0; comp < 2; ++comp) {
@@ -347,10 +292,12 @@ __device__ __forceinline__ void pass(C2<T> *vals, int tid, const Lines &ln, int 
                 }
                 __syncthreads();
             }
-            pass<T, L, G, ROWS, P + 1, Sink, true, true>(vals, tid, ln, ts, tw, lds, gin, sink);
+            pass<T, L, G, ROWS, P + 1, Sink, true, true, E>(vals, tid, ln, ts, tw, lds, gin,
+                                                            sink);
         } else {
             if constexpr (to_lds) __syncthreads();
-            pass<T, L, G, ROWS, P + 1, Sink, false, SPLIT>(vals, tid, ln, ts, tw, lds, gin, sink);
+            pass<T, L, G, ROWS, P + 1, Sink, false, SPLIT, E>(vals, tid, ln, ts, tw, lds, gin,
+                                                              sink);
         }
     }
 }
@@ -512,10 +459,14 @@ k_fft_cols(ColArgs a, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gou
 // every w-plane in turn and accumulates the phase-rotated results of each
 // output pixel in registers; only the image is written.  Output element n of
 // set s is image row  s*vs_mul + n*v_stride.
-template <typename T>
+template <typename T, int E>
 struct PhaseSink {
-    double are[kE], aim[kE];
-    double nfv[kE];       // FP64: n of each of this thread's pixels (set once)
+    static constexpr bool kRecur = sizeof(T) == 8 && kPhaseRecurF64;
+    double are[E], aim[E];
+    // FP64: n of each of this thread's pixels (set once), or, with the
+    // recurrence, each pixel's current rotation and its per-plane step.
+    double nfv[kRecur ? 1 : E];
+    double rc[kRecur ? E : 1], rs[kRecur ? E : 1], sc[kRecur ? E : 1], ss[kRecur ? E : 1];
     const double *nfl;    // FP32: n of the workgroup's pixels in LDS, [n * G + g]
     int G;
     const double *ltab, *mtab;
@@ -530,7 +481,14 @@ struct PhaseSink {
             aim[idx] += (double)o.y;
             return;
         }
-        if constexpr (sizeof(T) == 8) {
+        if constexpr (kRecur) {
+            // (p + iq) * exp(i arg_w); dwnorm * n is applied once at the end.
+            const double r = rc[idx], q = rs[idx];
+            are[idx] += (double)o.x * r - (double)o.y * q;
+            aim[idx] += (double)o.x * q + (double)o.y * r;
+            rc[idx] = r * sc[idx] - q * ss[idx];
+            rs[idx] = r * ss[idx] + q * sc[idx];
+        } else if constexpr (sizeof(T) == 8) {
             phase_add_f64(o.x, o.y, wterm, nfv[idx], dwnorm, are[idx], aim[idx]);
         } else {
             phase_add_f32(o.x, o.y, wterm, nfl[n * G + g], dwnorm, are[idx], aim[idx]);
@@ -538,14 +496,14 @@ struct PhaseSink {
     }
 };
 
-template <typename T, int L>
-__global__ void __launch_bounds__(256)
+template <typename T, int L, int E>
+__global__ void __launch_bounds__(lines_per_wg(L) * L / E)
 k_fft_cols_phase(ColArgs a, int planes, const C2<T> *__restrict__ tw,
                  const C2<T> *__restrict__ grid, PhaseGeom pg, int vs_mul, int v_stride,
                  double *__restrict__ ore, double *__restrict__ oim, ColOcc co)
 {
     constexpr int G = lines_per_wg(L);
-    constexpr int NT = G * L / kE;
+    constexpr int NT = G * L / E;
     __shared__ C2<T> lds[lds_elems(L, false)];
     // Band flags of two planes: with the FP64 prefetch, plane iw + 1's flags
     // are written while other waves may still be choosing which of plane
@@ -554,9 +512,9 @@ k_fft_cols_phase(ColArgs a, int planes, const C2<T> *__restrict__ tw,
     __shared__ double nfl[sizeof(T) == 4 ? G * L : 1];
     const int c0 = blockIdx.x * G;
     const int s = blockIdx.y;
-    PhaseSink<T> sink;
+    PhaseSink<T, E> sink;
 #pragma unroll
-    for (int i = 0; i < kE; ++i) sink.are[i] = sink.aim[i] = 0.0;
+    for (int i = 0; i < E; ++i) sink.are[i] = sink.aim[i] = 0.0;
     sink.ltab = pg.ltab;
     sink.mtab = pg.mtab;
     sink.dwnorm = pg.dwnorm;
@@ -576,8 +534,8 @@ k_fft_cols_phase(ColArgs a, int planes, const C2<T> *__restrict__ tw,
     ln.rowlive = nullptr;
     ln.out_base = 0;
     // Pixels of this thread in the last pass's element layout.
-    constexpr int P = npasses(L) - 1, R = radix_of(L, P), Ns = ns_of(L, P), LR = L / R,
-                  BPT = kE / R;
+    constexpr int P = npasses(L, E) - 1, R = radix_of(L, P, E), Ns = ns_of(L, P, E), LR = L / R,
+                  BPT = E / R;
     if constexpr (sizeof(T) == 4) {
         // FP32: n of every pixel once, in LDS (registers are kept for two
         // waves per SIMD).
@@ -599,9 +557,23 @@ k_fft_cols_phase(ColArgs a, int planes, const C2<T> *__restrict__ tw,
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int n = dbase + r * Ns;
-                sink.nfv[i * R + r] =
+                const double nf =
                     g < sink.nvalid ? pixel_n(pg.ltab[c0 + g], pg.mtab[s * vs_mul + n * v_stride])
                                     : 1.0;
+                if constexpr (PhaseSink<T, E>::kRecur) {
+                    // Plane 0's argument is the reference's (wterm = wmin,
+                    // phase_correction.cu:200, 222); each plane adds dw.
+                    double a0, a1;
+                    {
+#pragma clang fp contract(off)
+                        a0 = 2.0 * kRefPi * pg.wmin * (nf - 1.0);
+                        a1 = 2.0 * kRefPi * pg.dw * (nf - 1.0);
+                    }
+                    rick_sincos(a0, &sink.rs[i * R + r], &sink.rc[i * R + r]);
+                    rick_sincos(a1, &sink.ss[i * R + r], &sink.sc[i * R + r]);
+                } else {
+                    sink.nfv[i * R + r] = nf;
+                }
             }
         }
     }
@@ -613,28 +585,29 @@ k_fft_cols_phase(ColArgs a, int planes, const C2<T> *__restrict__ tw,
     // FP64 runs one wave per SIMD at this register count: the next plane's
     // inputs are loaded into registers while this plane is transformed.  (The
     // FP32 variant keeps two waves per SIMD instead, which measured faster.)
-    constexpr bool kPrefetch = sizeof(T) == 8;
-    C2<T> vals[kE];
+    constexpr bool kPrefetch = sizeof(T) == 8 && kPhasePrefetchF64;
+    C2<T> vals[E];
     if constexpr (kPrefetch) {
         plane_lines(0);
-        load_first<T, L, G, false>(vals, threadIdx.x, ln, grid);
+        load_first<T, L, G, false, E>(vals, threadIdx.x, ln, grid);
     }
     for (int iw = 0; iw < planes; ++iw) {
         sink.wterm = pg.wmin + iw * pg.dw;  // phase_correction.cu:200
         if constexpr (kPrefetch) {
-            C2<T> nxt[kE];
+            C2<T> nxt[E];
             if (iw + 1 < planes) {
                 plane_lines(iw + 1);
-                load_first<T, L, G, false>(nxt, threadIdx.x, ln, grid);
+                load_first<T, L, G, false, E>(nxt, threadIdx.x, ln, grid);
             }
-            pass<T, L, G, false, 0, PhaseSink<T>, true>(vals, threadIdx.x, ln, a.ts, tw, lds,
-                                                         grid, sink);
+            pass<T, L, G, false, 0, PhaseSink<T, E>, true, false, E>(vals, threadIdx.x, ln, a.ts,
+                                                                    tw, lds, grid, sink);
             __syncthreads();  // LDS is reused by the next plane
 #pragma unroll
-            for (int e = 0; e < kE; ++e) vals[e] = nxt[e];
+            for (int e = 0; e < E; ++e) vals[e] = nxt[e];
         } else {
             plane_lines(iw);
-            pass<T, L, G, false, 0>(vals, threadIdx.x, ln, a.ts, tw, lds, grid, sink);
+            pass<T, L, G, false, 0, PhaseSink<T, E>, false, false, E>(vals, threadIdx.x, ln,
+                                                                     a.ts, tw, lds, grid, sink);
             __syncthreads();  // LDS is reused by the next plane
         }
     }
@@ -649,8 +622,16 @@ k_fft_cols_phase(ColArgs a, int planes, const C2<T> *__restrict__ tw,
             const int n = dbase + r * Ns;
             if (g < sink.nvalid) {
                 const size_t pix = (size_t)(s * vs_mul + n * v_stride) * a.nx + (c0 + g);
-                ore[pix] = sink.are[i * R + r];
-                oim[pix] = sink.aim[i * R + r];
+                double vre = sink.are[i * R + r], vim = sink.aim[i * R + r];
+                if constexpr (PhaseSink<T, E>::kRecur) {
+                    if (sink.phase) {
+                        const double nf = pixel_n(pg.ltab[c0 + g], pg.mtab[s * vs_mul + n * v_stride]);
+                        vre = vre * pg.dwnorm * nf;
+                        vim = vim * pg.dwnorm * nf;
+                    }
+                }
+                ore[pix] = vre;
+                oim[pix] = vim;
             }
         }
     }
@@ -703,8 +684,9 @@ hipError_t launch_cols_phase(int L, const ColArgs &a, int sets, int planes, cons
 #define RICK_CPH(LL)                                                                         \
     case LL: {                                                                               \
         constexpr int G = lines_per_wg(LL);                                                  \
-        hipLaunchKernelGGL((k_fft_cols_phase<T, LL>), dim3((a.nx + G - 1) / G, sets, 1),     \
-                           dim3(G * LL / kE), 0, s, a, planes, tw, grid, pg, vs_mul,         \
+        constexpr int E = phase_values<T>(LL);                                               \
+        hipLaunchKernelGGL((k_fft_cols_phase<T, LL, E>), dim3((a.nx + G - 1) / G, sets, 1),  \
+                           dim3(G * LL / E), 0, s, a, planes, tw, grid, pg, vs_mul,          \
                            v_stride, re, im, co);                                            \
         return hipGetLastError();                                                            \
     }

@@ -65,6 +65,7 @@ constexpr int kBinHashBits = 13;
 constexpr int kBinSlots = 1 << kBinHashBits;
 constexpr int kMaxProbe = 32;
 constexpr uint32_t kZeroTask = 2u;            // task.w of a zero task
+constexpr uint32_t kSplitTask = 3u;           // task.w of a split bucket's task: 3 + split slot
 constexpr int kTileThreads = 512;
 constexpr int kLdsBytes = 160 * 1024;         // LDS per CU
 constexpr uint32_t kPersistSlots = 64;        // task slots per resident workgroup to persist
@@ -341,6 +342,7 @@ k_scan_top(GridGeom g, uint32_t *__restrict__ blocksum, uint32_t nscan,
         meta[3] = carry[3];             // non-empty buckets
         meta[4] = carry[1];             // accumulation tasks (stored first)
         meta[5] = 0;                    // next task for the tile kernel's workgroups
+        meta[6] = meta[7] = 0;          // largest |S| of the call (FP32 split buckets)
     }
 }
 
@@ -399,9 +401,12 @@ k_scan_down(GridGeom g, const uint32_t *__restrict__ counts,
         } else {
             run = 0;
             const uint32_t nt = ntasks_of(c[k], g.rmax);
+            // Tasks of a split bucket carry its slot in split_list (always
+            // below split_cap: a split bucket holds more than rmax records).
+            const uint32_t tag = nt > 1 ? (so < split_cap ? kSplitTask + so : 1u) : 0u;
             for (uint32_t q = 0; q < nt; ++q) {
                 const uint32_t r0 = ro + q * g.rmax, r1 = min(r0 + g.rmax, ro + c[k]);
-                if (to + q < task_cap) tasks[to + q] = make_uint4(b, r0, r1, nt > 1 ? 1u : 0u);
+                if (to + q < task_cap) tasks[to + q] = make_uint4(b, r0, r1, tag);
             }
             if (nt > 1 && so < split_cap) split_list[so++] = b;
             to += nt;
@@ -484,9 +489,10 @@ k_scatter(GridGeom g, uint32_t n, uint32_t F, uint32_t P, const double *__restri
           const double *__restrict__ vv, const double *__restrict__ ww,
           const float *__restrict__ vr, const float *__restrict__ vi,
           const float *__restrict__ wt, uint32_t *__restrict__ cursor,
-          typename RecOf<PREC>::type *__restrict__ rec)
+          typename RecOf<PREC>::type *__restrict__ rec, uint32_t *__restrict__ meta)
 {
     __shared__ uint32_t hkey[kHashSlots], hcnt[kHashSlots], hbase[kHashSlots];
+    __shared__ unsigned long long smax[kScatterThreads / 64];
     for (int i = threadIdx.x; i < kHashSlots; i += kScatterThreads) {
         hkey[i] = kEmpty;
         hcnt[i] = 0;
@@ -569,7 +575,27 @@ k_scatter(GridGeom g, uint32_t n, uint32_t F, uint32_t P, const double *__restri
             }
         }
     }
+    if constexpr (PREC == RICK_FP32) {
+        // Largest |S| (as stored, FP32) of the block's rows: bit patterns of
+        // non-negative values order like the values, NaN above Inf.
+        unsigned long long m = 0ull;
+#pragma unroll
+        for (int k = 0; k < kRowsPerThread; ++k) {
+            if (!nk[k]) continue;
+            m = max(m, (unsigned long long)__double_as_longlong(fabs((double)(float)sre[k])));
+            m = max(m, (unsigned long long)__double_as_longlong(fabs((double)(float)sim[k])));
+        }
+        for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned long long)__shfl_xor(m, o));
+        if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = m;
+    }
     __syncthreads();
+    if constexpr (PREC == RICK_FP32) {
+        if (threadIdx.x == 0) {
+            unsigned long long m = 0ull;
+            for (int w = 0; w < kScatterThreads / 64; ++w) m = max(m, smax[w]);
+            if (m) atomicMax(reinterpret_cast<unsigned long long *>(meta + 6), m);
+        }
+    }
     // One returning global atomic per distinct key, all issued together.
     for (int i = threadIdx.x; i < kHashSlots; i += kScatterThreads) {
         const uint32_t c = hcnt[i];
@@ -624,6 +650,27 @@ k_zero_split(GridGeom g, const uint32_t *__restrict__ split_list, const uint32_t
 }
 
 
+// FP32 split buckets (several tasks add into one tile) accumulate in one
+// 64-bit fixed-point tile per bucket with integer atomics, so the sum does not
+// depend on which records a task holds or the order tasks finish in; the last
+// task of the bucket rounds the tile to FP32 once.  Every task must use the
+// same scale, from a bound all of them form alike: (records in the bucket) x
+// (largest |S| of the whole call, an integer max over the bit patterns of
+// non-negative values, taken by k_scatter) x (largest cell weight), mapped
+// below 2^50.  Returns -1 when the bound is not finite (NaN/Inf data): the
+// bucket's tasks then add FP32 atomically into the (cleared) grid tile.
+__device__ __forceinline__ double split_scale_of(const GridGeom &g, unsigned long long maxbits,
+                                                 uint32_t count)
+{
+    const double m = __longlong_as_double((long long)maxbits);
+    const double bound = (double)count * m * g.cmax * 1.01;   // margin for FP32 products
+    if (!(isfinite(m) && isfinite(bound))) return -1.0;
+    if (!(bound > 0.0)) return 1.0;
+    int e;
+    frexp(bound, &e);                                          // bound < 2^e
+    return ldexp(1.0, 50 - e);
+}
+
 // 2^52 + 2^51: adding it to x with |x| < 2^51 leaves round(x) in the low
 // mantissa bits, so bits(x + M) - bits(M) is round(x) as a two's-complement
 // integer (one FMA and one integer subtraction per value).
@@ -661,7 +708,8 @@ template <int PREC, int TU, int TV, int SMAX, bool HIPREC>
 __global__ void __launch_bounds__(kTileThreads)
 k_tile(GridGeom g, const uint4 *__restrict__ tasks, uint32_t *__restrict__ meta,
        const typename RecOf<PREC>::type *__restrict__ rec, const double *__restrict__ table,
-       void *grid_v, int accumulate)
+       void *grid_v, int accumulate, const uint32_t *__restrict__ offsets,
+       uint32_t *__restrict__ split_ticket, unsigned long long *__restrict__ split_acc)
 {
     using T = typename std::conditional<PREC == RICK_FP64, double, float>::type;
     using T2 = typename std::conditional<PREC == RICK_FP64, double2, float2>::type;
@@ -676,7 +724,7 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, uint32_t *__restrict__ meta,
     // else zero task Tr + i - f(i): the two kinds are spread evenly so that
     // clearing overlaps accumulation.
     const uint32_t ntask = meta[1], nreal = meta[4];
-    __shared__ uint32_t slot_sh;
+    __shared__ uint32_t slot_sh, last_sh;
     uint32_t slot = blockIdx.x;
     for (bool first = true;; first = false) {
         if (!first) {
@@ -695,7 +743,17 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, uint32_t *__restrict__ meta,
         const uint32_t fn = (uint32_t)((uint64_t)(slot + 1) * nreal / ntask);
         const uint4 task = tasks[fn > fi ? fi : nreal + slot - fi];
         const uint32_t b = task.x, r0 = task.y, r1 = task.z;
-        const bool split = task.w == 1u;
+        const bool split = task.w == 1u || task.w >= kSplitTask;
+        // FP32 split bucket with a finite bound: the bucket-wide scale, and the
+        // tile goes to its integer accumulator (k_split_prep / k_split_finish).
+        const uint32_t slot = task.w >= kSplitTask ? task.w - kSplitTask : 0u;
+        const double bucket_scale =
+            PREC == RICK_FP32 && task.w >= kSplitTask
+                ? split_scale_of(g,
+                                 *reinterpret_cast<const unsigned long long *>(meta + 6),
+                                 offsets[b + 1] - offsets[b])
+                : 0.0;
+        const bool det = bucket_scale > 0.0;
 
         const uint32_t per_plane = (uint32_t)g.tiles_u * (uint32_t)g.tiles_v;
         const int w = (int)(b / per_plane), rem = (int)(b % per_plane);
@@ -743,7 +801,10 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, uint32_t *__restrict__ meta,
         // FP32: fixed-point scale from the bound of the task's cell sums.
         bool fixed = false;
         double scale = 1.0;
-        if constexpr (PREC == RICK_FP32) {
+        if (PREC == RICK_FP32 && det) {
+            fixed = true;
+            scale = bucket_scale;
+        } else if constexpr (PREC == RICK_FP32) {
             __shared__ double red[2][kTileThreads / 64];
             double ar = 0.0, ai = 0.0;
             for (uint32_t r = r0 + threadIdx.x; r < r1; r += kTileThreads) {
@@ -878,6 +939,54 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, uint32_t *__restrict__ meta,
         }
         __syncthreads();
 
+        if (PREC == RICK_FP32 && det) {
+            unsigned long long *acc = split_acc + (size_t)slot * 2 * (TU * TV);
+            for (int idx = threadIdx.x; idx < nk * TU; idx += kTileThreads) {
+                const int kk = idx / TU, jj = idx % TU;
+                if (jj >= nj) continue;
+                const int li = kk * LDU + jj;
+                const unsigned long long a = reinterpret_cast<const unsigned long long *>(sre)[li];
+                const unsigned long long b2 = reinterpret_cast<const unsigned long long *>(sim)[li];
+                if (a) atomicAdd(&acc[idx], a);
+                if (b2) atomicAdd(&acc[TU * TV + idx], b2);
+            }
+            // The bucket's last task to get here writes the tile and clears
+            // the accumulator and ticket for the next call.
+            __threadfence();
+            __syncthreads();
+            if (threadIdx.x == 0)
+                last_sh = atomicAdd(&split_ticket[slot], 1u) ==
+                          ntasks_of(offsets[b + 1] - offsets[b], g.rmax) - 1u;
+            __syncthreads();
+            if (last_sh) {
+                __threadfence();
+                const double inv = 1.0 / bucket_scale;   // exact: a power of two
+                T2 *g2 = reinterpret_cast<T2 *>(grid);
+                for (int idx = threadIdx.x; idx < nk * TU; idx += kTileThreads) {
+                    const int kk = idx / TU, jj = idx % TU;
+                    if (jj >= nj) continue;
+                    const long long a = (long long)__hip_atomic_exchange(
+                        &acc[idx], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const long long b2 = (long long)__hip_atomic_exchange(
+                        &acc[TU * TV + idx], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    T2 v;
+                    v.x = (T)((double)a * inv);
+                    v.y = (T)((double)b2 * inv);
+                    const size_t c =
+                        ((size_t)w * g.ny + (size_t)(k0 + kk)) * (size_t)g.nx + (size_t)(j0 + jj);
+                    if (accumulate) {
+                        const T2 o = g2[c];
+                        v.x = o.x + v.x;
+                        v.y = o.y + v.y;
+                    }
+                    g2[c] = v;
+                }
+                if (threadIdx.x == 0)
+                    __hip_atomic_store(&split_ticket[slot], 0u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+            continue;
+        }
         for (int idx = threadIdx.x; idx < nk * TU; idx += kTileThreads) {
             const int kk = idx / TU, jj = idx % TU;
             if (jj >= nj) continue;
@@ -951,7 +1060,8 @@ hipError_t launch_tile_s(const GridGeom &g, const GridWork &w, const double *tab
 #define RICK_TILE_LAUNCH(A, B, HI)                                                          \
     hipLaunchKernelGGL((k_tile<PREC, A, B, SMAX, HI>),                                      \
                        dim3(tile_grid<PREC, A, B, SMAX, HI>(w.task_cap)), blk, 0, s,        \
-                       g, w.tasks, w.meta, rec, table, grid, accumulate)
+                       g, w.tasks, w.meta, rec, table, grid, accumulate, w.offsets,        \
+                       w.split_ticket, w.split_acc)
 #define RICK_TILE_CASE(A, B)                                                                \
     if (g.tu == A && g.tv == B) {                                                           \
         tm->begin(RICK_STAGE_TILE, s);                                                      \
@@ -972,14 +1082,19 @@ hipError_t launch_tile_s(const GridGeom &g, const GridWork &w, const double *tab
     return hipErrorInvalidValue;
 }
 
-// Footprint loops are unrolled to 7 (supports 1-7), 9, or 15 (11-15).
+// Footprint loops are unrolled to g.smax: 7, 9, 11 or 15 (>= the support;
+// the plan picks the smallest unless told otherwise).
 template <int PREC>
 hipError_t launch_tile(const GridGeom &g, const GridWork &w, const double *table, void *grid,
                        int accumulate, hipStream_t s, Timing *tm)
 {
-    if (g.support <= 7) return launch_tile_s<PREC, 7>(g, w, table, grid, accumulate, s, tm);
-    if (g.support <= 9) return launch_tile_s<PREC, 9>(g, w, table, grid, accumulate, s, tm);
-    return launch_tile_s<PREC, 15>(g, w, table, grid, accumulate, s, tm);
+    switch (g.smax) {
+        case 7: return launch_tile_s<PREC, 7>(g, w, table, grid, accumulate, s, tm);
+        case 9: return launch_tile_s<PREC, 9>(g, w, table, grid, accumulate, s, tm);
+        case 11: return launch_tile_s<PREC, 11>(g, w, table, grid, accumulate, s, tm);
+        case 15: return launch_tile_s<PREC, 15>(g, w, table, grid, accumulate, s, tm);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -1031,6 +1146,10 @@ size_t grid_work_bytes(const GridGeom &g, uint32_t max_points, int precision)
     b += align_up(sizeof(uint32_t) * 8, 256);              // meta
     b += align_up(rs * rec_cap, 256);                      // records
     b += align_up(sizeof(uint32_t) * (size_t)g.planes * g.tiles_v, 256);  // band
+    if (precision == RICK_FP32) {
+        b += align_up(sizeof(uint32_t) * split_cap, 256);                     // split_ticket
+        b += align_up(sizeof(unsigned long long) * 2 * (size_t)g.tu * g.tv * split_cap, 256);
+    }
     return b;
 }
 
@@ -1055,6 +1174,13 @@ void grid_work_carve(const GridGeom &g, uint32_t max_points, int precision, void
     w->meta = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * 8));
     w->records = take(rs * rec_cap);
     w->band = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * (size_t)g.planes * g.tiles_v));
+    w->split_ticket = nullptr;
+    w->split_acc = nullptr;
+    if (precision == RICK_FP32) {
+        w->split_ticket = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * split_cap));
+        w->split_acc = reinterpret_cast<unsigned long long *>(
+            take(sizeof(unsigned long long) * 2 * (size_t)g.tu * g.tv * split_cap));
+    }
     w->rec_cap = rec_cap;
     w->split_cap = split_cap;
     w->task_cap = task_cap;
@@ -1099,7 +1225,7 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
 #define RICK_SCATTER(PR, VM)                                                                  \
     hipLaunchKernelGGL((k_scatter<PR, VM>), dim3(nblk), dim3(kScatterThreads), 0, s, g, n, F, P, \
                        uu, vv, ww, vr, vi, wt, w.cursor,                                      \
-                       static_cast<typename RecOf<PR>::type *>(w.records))
+                       static_cast<typename RecOf<PR>::type *>(w.records), w.meta)
 #define RICK_SCATTER_MODES(PR)                                                                \
     switch (mode) {                                                                           \
     case 4: RICK_SCATTER(PR, 4); break;                                                       \
@@ -1116,19 +1242,24 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
 #undef RICK_SCATTER
     }
     tm->end(RICK_STAGE_SCATTER, s);
-    if (!accumulate) {
-        tm->begin(RICK_STAGE_ZERO_SPLIT, s);
-        if (precision == RICK_FP64)
+    if (precision == RICK_FP64) {
+        if (!accumulate) {
+            tm->begin(RICK_STAGE_ZERO_SPLIT, s);
             hipLaunchKernelGGL(k_zero_split<double>, dim3(w.split_cap), dim3(kThreads), 0, s, g,
                                w.split_list, w.meta, static_cast<double *>(grid));
-        else
-            hipLaunchKernelGGL(k_zero_split<float>, dim3(w.split_cap), dim3(kThreads), 0, s, g,
-                               w.split_list, w.meta, static_cast<float *>(grid));
+            tm->end(RICK_STAGE_ZERO_SPLIT, s);
+        }
+        return launch_tile<RICK_FP64>(g, w, table_d, grid, accumulate, s, tm);
+    }
+    // FP32: split buckets accumulate in integer fixed point (order-independent);
+    // their grid tiles are cleared for the non-finite fallback.
+    if (!accumulate) {
+        tm->begin(RICK_STAGE_ZERO_SPLIT, s);
+        hipLaunchKernelGGL(k_zero_split<float>, dim3(w.split_cap), dim3(kThreads), 0, s, g,
+                           w.split_list, w.meta, static_cast<float *>(grid));
         tm->end(RICK_STAGE_ZERO_SPLIT, s);
     }
-    return precision == RICK_FP64
-               ? launch_tile<RICK_FP64>(g, w, table_d, grid, accumulate, s, tm)
-               : launch_tile<RICK_FP32>(g, w, table_d, grid, accumulate, s, tm);
+    return launch_tile<RICK_FP32>(g, w, table_d, grid, accumulate, s, tm);
 }
 
 }  // namespace rick

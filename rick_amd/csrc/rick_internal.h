@@ -67,6 +67,7 @@ struct GridGeom {
     int ntable;
     double cmax;            // largest per-cell convolution weight (a bound)
     int zrun;               // empty buckets cleared per zero task (1..4)
+    int smax;               // unrolled footprint length of the tile kernel (>= support)
 };
 
 // Workspace of the binned gridder, carved out of one device allocation.
@@ -83,6 +84,12 @@ struct GridWork {
                             //   tile kernel's next task slot (persistent workgroups)
     void *records;          // [rec_cap] sorted row records
     uint32_t *band;         // [planes*tiles_v] tile rows holding an occupied tile
+    // FP32 only: per split slot, the count of its tasks done and its 64-bit
+    // fixed-point accumulator tile (re then im, tu*tv each); both are zero
+    // between calls (cleared at plan creation, then by the bucket's last
+    // task).  Null for FP64.
+    uint32_t *split_ticket;         // [split_cap]
+    unsigned long long *split_acc;  // [split_cap * 2 * tu * tv]
     uint32_t rec_cap, split_cap, task_cap, nscan;
 };
 
@@ -110,6 +117,37 @@ struct FftPlan {
 bool fft_needs_tmp(int precision, int ny);
 hipError_t launch_fft(const FftPlan &p, int precision, const void *in, void *out, void *tmp,
                       double scale, hipStream_t s, Timing *tm);
+
+// FFT of any axis lengths >= 2 (mixed radix 2..16, 3, 5, 7, 11, 13 in
+// LDS; four-step for long or strided axes; Bluestein for other primes).
+constexpr int kGenMaxPasses = 16;
+struct GenPassDesc {
+    int L, np;                      // line length and its passes
+    int rad[kGenMaxPasses];         // radix of each pass
+};
+struct GenAxis {
+    int n;                          // axis length
+    int mode;                       // 0 direct, 1 four-step, 2 Bluestein
+    int n1, n2;                     // four-step split n = n1*n2
+    GenPassDesc a, b;               // direct: a; four-step: a (n1), b (n2)
+    int m, mmode, m1, m2;           // Bluestein: convolution length and its plan
+    GenPassDesc ma, mb;
+    const void *tw;                 // device exp(+2 pi i k/n) [n] (Bluestein: [m])
+    const void *chirp, *hf;         // Bluestein: exp(+i pi k^2/n) [n], F_m(chirp*) [m]
+};
+struct GenFft {
+    GenAxis ax[2];                  // 0: rows (length nx), 1: columns (length ny)
+    size_t table_bytes, work_bytes; // device bytes of the tables and of the scratch
+    void *work;
+};
+// Plans both axes; false if an axis is shorter than 2 or too long.
+bool gen_fft_prepare(int nx, int ny, int precision, GenFft *g);
+// Writes the tables (synchronously) at `tables` (table_bytes) and records the
+// scratch `work` (work_bytes).
+hipError_t gen_fft_upload(GenFft *g, int precision, void *tables, void *work);
+// Per-plane backward 2-D transform, in == out allowed, scaled.
+hipError_t launch_fft_gen(const GenFft &g, int precision, int planes, int nx, int ny,
+                          const void *in, void *out, double scale, hipStream_t s, Timing *tm);
 
 // Phase correction + plane sum.
 struct PhaseGeom {

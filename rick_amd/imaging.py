@@ -108,7 +108,7 @@ class Plan:
     def __init__(self, grid_x, grid_y, planes, kernel=KAISERBESSEL, support=7, dx=None,
                  dw=None, precision=FP64, max_points=0, resolution=1.0, wmin=0.0, wmax=1.0,
                  phase_on=True, tile=(0, 0), rows_per_task=0, device=0, xaxistot=0,
-                 yaxistot=0):
+                 yaxistot=0, unroll=0):
         d = _lib.PlanDesc()
         d.num_w_planes = planes
         d.grid_size_x = grid_x
@@ -127,6 +127,7 @@ class Plan:
         d.rows_per_task = rows_per_task
         d.device = device
         d.xaxistot, d.yaxistot = xaxistot, yaxistot   # a rank's slab of the image
+        d.footprint_unroll = unroll                    # 0: smallest of 7, 9, 11, 15 >= support
         self.desc = d
         self._h = ctypes.c_void_p()
         check(lib().rick_plan_create(ctypes.byref(self._h), ctypes.byref(d)))

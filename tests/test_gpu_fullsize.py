@@ -1,6 +1,9 @@
 """Full-size checks on the LOFAR-like workload (7,745,536 rows x 16 channels =
-123,928,576 visibilities onto 4096^2 x 32 w-planes, Kaiser-Bessel support 7),
-where the CPU oracle would take minutes: properties that hold at any size.
+123,928,576 visibilities onto 4096^2 x 32 w-planes, Kaiser-Bessel support 7)
+and the JVLA-like one (2,875,392 rows x 64 channels = 184,025,088
+visibilities onto 8192^2 x 64 w-planes, Kaiser-Bessel support 7).
+
+Properties that hold at any size:
 
 * Conservation through binning and tiling.  Every cell gets S*Ku[j]*Kv[k]
   from each row whose footprint covers it (w-stacking.cu:499-532).  So the
@@ -13,7 +16,17 @@ where the CPU oracle would take minutes: properties that hold at any size.
   transform) and the DC term (X[0,0] = sum x / (Nx*Ny)), per plane.
 * The fused image chain against the separate grid -> FFT -> phase path.
 * The FP32 image against the FP64 image: relative L2 <= 1e-5.
+
+And, on the same rows, against the CPU oracle run with every host core the
+process may use (at most 16): the FP64 grid within relative L2 1e-12 and
+max-abs 1e-13 of the largest cell, the FP64 image within 1e-12 and the FP32
+image within 1e-5 of the oracle's FFT + phase correction
+(w-stacking.cu:436-541, fourier_transform.c:13-93, phase_correction.cu:
+185-250).
 """
+import os
+import time
+
 import numpy as np
 import pytest
 import torch
@@ -24,6 +37,21 @@ from rick_amd import FP32, FP64, plan_for, synth
 pytestmark = pytest.mark.gpu
 
 KL_OVERSAMPLE = 5
+
+
+def host_threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def dev_rel(a, b):
+    """(relative L2, max-abs / max|b|) of two device tensors, in float64."""
+    d = (a.double() - b).abs()
+    return ((torch.linalg.norm(d) / torch.linalg.norm(b)).item(),
+            (d.max() / b.abs().max()).item())
 
 
 @pytest.fixture(scope="module")
@@ -98,13 +126,34 @@ def plane_sum_deviation(ds, grid):
     return np.abs(np.concatenate([got_re - want_re, got_im - want_im])).max() / scale
 
 
-@pytest.mark.parametrize("precision", [FP64, FP32])
-def test_fullsize_jvla_grid_past_32bit_indices(precision):
-    # 8192^2 x 64 planes: 2*planes*Nx*Ny = 8.6e9 grid values, past the 2^31
-    # the reference's int index reaches (68.7 GB in FP64).
+@pytest.fixture(scope="module")
+def jvla():
     cfg = synth.CONFIGS["jvla"]
-    print(f"case full-size {cfg.name} precision={precision}: {cfg.n_vis} vis", flush=True)
+    print(f"case full-size {cfg.name}: {cfg.n_rows} rows, {cfg.n_vis} vis", flush=True)
     ds = synth.make_dataset(cfg, device="cuda:0")
+    torch.cuda.synchronize()
+    yield ds
+    del ds
+    torch.cuda.empty_cache()
+
+
+def busiest_planes(ds, k=3):
+    """First of the k consecutive w-planes holding the most rows."""
+    plane = torch.trunc(ds.ww / ds.dw).long()
+    ok = (plane >= 0) & (plane < ds.cfg.planes)
+    counts = torch.bincount(plane[ok], minlength=ds.cfg.planes).cpu().numpy()
+    run = np.convolve(counts, np.ones(k, dtype=np.int64), mode="valid")
+    return int(np.argmax(run))
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32])
+def test_fullsize_jvla_grid_past_32bit_indices(jvla, precision):
+    # 8192^2 x 64 planes: 2*planes*Nx*Ny = 8.6e9 grid values, past the 2^31
+    # the reference's int index reaches (68.7 GB in FP64).  Three w-planes
+    # are compared with the CPU oracle gridding the rows of those planes and
+    # their neighbours (a row only writes its own plane).
+    ds, cfg = jvla, jvla.cfg
+    print(f"case full-size {cfg.name} grid precision={precision}", flush=True)
     plan = plan_for(ds, precision=precision)
     grid = plan.grid(ds.uu, ds.vv, ds.ww, ds.vis_real, ds.vis_img, ds.weight, cfg.n_chan,
                      cfg.n_pol)
@@ -113,7 +162,64 @@ def test_fullsize_jvla_grid_past_32bit_indices(precision):
     last = grid.view(cfg.planes, -1)[-1].abs().max().item()
     print(f"plane sums: max deviation {err:.3e}; last plane peak {last:.3e}", flush=True)
     assert err <= (1e-10 if precision == FP64 else 1e-5)
-    del grid, plan, ds
+    p0, k = busiest_planes(ds), 3
+    plane = torch.trunc(ds.ww / ds.dw)
+    sel = ((plane >= p0 - 1) & (plane < p0 + k + 1)).nonzero().squeeze(1)
+    F, P = cfg.n_chan, cfg.n_pol
+    vsel = (sel[:, None] * (F * P) + torch.arange(F * P, device=sel.device)[None, :]).reshape(-1)
+    wsel = (sel[:, None] * P + torch.arange(P, device=sel.device)[None, :]).reshape(-1)
+    host = lambda t: t.cpu().numpy()  # noqa: E731
+    # np.zeros maps zero pages lazily: only the window's planes become resident.
+    grid_o = O.wstack(cfg.kernel, cfg.planes, host(ds.uu[sel]), host(ds.vv[sel]),
+                      host(ds.ww[sel]), host(ds.vis_real[vsel]), host(ds.vis_img[vsel]),
+                      host(ds.weight[wsel]), F, P, ds.dx, ds.dw, cfg.support, cfg.grid,
+                      cfg.grid, threads=host_threads())
+    n = 2 * cfg.grid * cfg.grid
+    want = torch.from_numpy(grid_o[p0 * n:(p0 + k) * n]).to("cuda")
+    got = grid[p0 * n:(p0 + k) * n]
+    rel, peak = dev_rel(got, want)
+    print(f"planes {p0}..{p0 + k - 1} ({sel.numel()} rows in the window) vs oracle: rel-L2 "
+          f"{rel:.3e}, max-abs {peak:.3e} of the peak", flush=True)
+    assert want.abs().max().item() > 0
+    if precision == FP64:
+        assert rel <= 1e-12 and peak <= 1e-13
+    else:
+        assert rel <= 1e-6
+    del grid, plan, want, got
+    torch.cuda.empty_cache()
+
+
+def test_fullsize_jvla_chain(jvla):
+    # The whole 8192^2 x 64 chain: Parseval and DC of the 8192^2 FFT per
+    # plane, then the fused image chain in FP64 and FP32.
+    ds, cfg = jvla, jvla.cfg
+    ins = (ds.uu, ds.vv, ds.ww, ds.vis_real, ds.vis_img, ds.weight, cfg.n_chan, cfg.n_pol)
+    print(f"case full-size {cfg.name} chain", flush=True)
+    plan = plan_for(ds, precision=FP64)
+    grid = plan.grid(*ins)
+    n = cfg.grid * cfg.grid
+    g = grid.view(cfg.planes, n, 2)
+    # Per-plane reductions without a grid-sized temporary (68.7 GB here).
+    energy_in = torch.linalg.vector_norm(g, dim=(1, 2)) ** 2
+    sum_in = g.sum(dim=1)
+    plan.fft(grid)                                          # in place
+    energy_out = torch.linalg.vector_norm(g, dim=(1, 2)) ** 2
+    live = energy_in > 0
+    rel = ((energy_out * n - energy_in).abs()[live] / energy_in[live]).max().item()
+    dc = ((g[:, 0, :] * n - sum_in).abs().max() / sum_in.abs().max()).item()
+    print(f"8192^2 FFT: Parseval max rel {rel:.3e}, DC max rel {dc:.3e}", flush=True)
+    assert rel <= 1e-10 and dc <= 1e-10
+    re64, im64, _ = plan.image(*ins, grid=grid)
+    a = torch.cat([re64, im64])
+    del grid, g, plan
+    torch.cuda.empty_cache()
+    plan32 = plan_for(ds, precision=FP32)
+    re32, im32, _ = plan32.image(*ins)
+    rel32, peak32 = dev_rel(torch.cat([re32, im32]), a)
+    print(f"FP32 vs FP64 image: rel-L2 {rel32:.3e}, peak pixel {peak32:.3e}", flush=True)
+    assert torch.isfinite(a).all()
+    assert rel32 <= 1e-5
+    del plan32
     torch.cuda.empty_cache()
 
 
@@ -184,3 +290,47 @@ def test_fullsize_dropin_accumulates_into_host_grid(lofar):
     two = float(np.linalg.norm(grid - dev) / np.linalg.norm(dev))
     print(f"drop-in vs device grid {one:.3e}; after a second call vs twice {two:.3e}", flush=True)
     assert one <= 1e-13 and two <= 1e-13
+
+
+def test_fullsize_matches_cpu_oracle(lofar):
+    cfg = lofar.cfg
+    th = host_threads()
+    print(f"case full-size oracle parity: {cfg.n_vis} vis, {th} host threads", flush=True)
+    h = lofar.to_numpy()
+    t0 = time.perf_counter()
+    grid_o = O.wstack(cfg.kernel, cfg.planes, h.uu, h.vv, h.ww, h.vis_real, h.vis_img,
+                      h.weight, cfg.n_chan, cfg.n_pol, lofar.dx, lofar.dw, cfg.support,
+                      cfg.grid, cfg.grid, threads=th)
+    t_grid = time.perf_counter() - t0
+    ins = (lofar.uu, lofar.vv, lofar.ww, lofar.vis_real, lofar.vis_img, lofar.weight,
+           cfg.n_chan, cfg.n_pol)
+    plan = plan_for(lofar, precision=FP64)
+    grid = plan.grid(*ins)
+    go = torch.from_numpy(grid_o).to("cuda")
+    g_rel, g_max = dev_rel(grid, go)
+    del go
+    print(f"FP64 grid vs oracle: rel-L2 {g_rel:.3e}, max-abs {g_max:.3e} of the peak "
+          f"(oracle wstack {t_grid:.1f} s)", flush=True)
+    t0 = time.perf_counter()
+    gridss = O.fft_planes(cfg.planes, cfg.grid, cfg.grid, grid_o, threads=th)
+    del grid_o
+    t_fft = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ore, oim = O.phase_correction(gridss, cfg.grid, cfg.grid, cfg.planes, lofar.resolution,
+                                  lofar.wmin, lofar.wmax, threads=th)
+    t_phase = time.perf_counter() - t0
+    del gridss
+    ref = torch.from_numpy(np.concatenate([ore, oim])).to("cuda")
+    re64, im64, _ = plan.image(*ins, grid=grid)
+    i64 = dev_rel(torch.cat([re64, im64]), ref)
+    del grid, plan
+    torch.cuda.empty_cache()
+    plan32 = plan_for(lofar, precision=FP32)
+    re32, im32, _ = plan32.image(*ins)
+    i32 = dev_rel(torch.cat([re32, im32]), ref)
+    print(f"FP64 image vs oracle: rel-L2 {i64[0]:.3e}, peak {i64[1]:.3e}; FP32 image vs "
+          f"oracle: rel-L2 {i32[0]:.3e}, peak {i32[1]:.3e} (oracle FFT {t_fft:.1f} s, phase "
+          f"{t_phase:.1f} s)", flush=True)
+    assert g_rel <= 1e-12 and g_max <= 1e-13
+    assert i64[0] <= 1e-12
+    assert i32[0] <= 1e-5

@@ -142,6 +142,18 @@ def test_grid_scattered_rows_overflow_block_tables(precision, F, P):
     assert st["records"] >= n - n // 97 - 1
 
 
+@pytest.mark.parametrize("name,unroll", [("wstack_kb_s7", 9), ("wstack_kb_s7", 11),
+                                         ("wstack_gauss_s9", 15), ("wstack_gausshp_s7", 11)])
+def test_grid_footprint_unroll_choices(name, unroll):
+    # Every footprint unroll the tuner may pick grids the golden vectors.
+    z = load_golden(f"{name}.npz")
+    plan = grid_plan(z, unroll=unroll)
+    got = host(plan.grid(*grid_inputs(z), int(z["F"]), int(z["P"]), out=nan_grid(plan)))
+    assert rel_l2(got, z["grid"]) <= FP64_TOL
+    with pytest.raises(rick_amd.RickError):
+        grid_plan(z, unroll=5)        # below the support / not instantiated
+
+
 @pytest.mark.parametrize("precision", [FP64, FP32])
 def test_grid_nan_imaginary_part_propagates(precision):
     """Only the real part is tested for NaN (w-stacking.cu:517): a NaN
@@ -231,6 +243,88 @@ def test_fft_matches_numpy(shape, precision, inplace):
     got = cplx(host(out), planes, ny, nx)
     ref = np.fft.ifft2(c, axes=(1, 2))
     assert rel_l2(got, ref) <= (1e-14 if precision == FP64 else 2e-6)
+
+
+# Axis lengths beside the power-of-two engine: mixed radix (2..16, 3, 5, 7,
+# 11, 13), four-step for long rows and columns, Bluestein for other primes
+# (FFTW plans any size, fourier_transform.c:34-36).
+@pytest.mark.parametrize("shape,planes", [
+    ((2000, 3000), 2),        # 2^4*5^3 x 2^3*3*5^3: four-step columns, direct rows of 3000
+    ((6000, 6000), 1),        # four-step on both axes
+    ((96, 128), 3),           # direct mixed radix
+    ((1013, 1009), 1),        # primes: Bluestein on both axes (M = 2048)
+    ((300, 4100), 1),         # 4100 = 4 * 5^2 * 41: Bluestein rows (M = 16384, four-step)
+    ((64, 16384), 1),         # power of two above 8192: four-step rows
+    ((7, 11), 2),             # short odd axes
+    ((2, 8), 1),
+])
+@pytest.mark.parametrize("precision", [FP64, FP32])
+@pytest.mark.parametrize("inplace", [False, True])
+def test_fft_any_size_matches_numpy(shape, planes, precision, inplace):
+    ny, nx = shape
+    print(f"case fft any-size {shape} planes={planes} prec={precision} inplace={inplace}",
+          flush=True)
+    rng = np.random.default_rng(ny * 7 + nx)
+    c = rng.normal(size=(planes, ny, nx)) + 1j * rng.normal(size=(planes, ny, nx))
+    flat = np.stack([c.real, c.imag], -1).reshape(-1)
+    plan = Plan(nx, ny, planes, precision=precision)
+    src = dev(flat if precision == FP64 else flat.astype(np.float32))
+    out = src if inplace else torch.empty_like(src)
+    plan.fft(src, out)
+    got = cplx(host(out), planes, ny, nx)
+    ref = np.fft.ifft2(c, axes=(1, 2))
+    err = rel_l2(got, ref)
+    print(f"rel-L2 vs numpy {err:.3e}", flush=True)
+    assert err <= (1e-14 if precision == FP64 else 2e-6)
+
+
+@pytest.mark.parametrize("shape", [(6, 10), (13, 17), (12, 7)])
+def test_fft_any_size_matches_brute_force_dft(shape):
+    # The defining sum, X[v,u] = 1/(Nx Ny) sum x[k,j] exp(+2 pi i (ju/Nx + kv/Ny)),
+    # in exact-index form (no FFT library involved).
+    ny, nx = shape
+    rng = np.random.default_rng(nx + 100 * ny)
+    c = rng.normal(size=(ny, nx)) + 1j * rng.normal(size=(ny, nx))
+    ku = np.exp(2j * np.pi * (np.outer(np.arange(nx), np.arange(nx)) % nx) / nx)
+    kv = np.exp(2j * np.pi * (np.outer(np.arange(ny), np.arange(ny)) % ny) / ny)
+    ref = kv @ c @ ku / (nx * ny)
+    plan = Plan(nx, ny, 1)
+    out = plan.fft(dev(np.stack([c.real, c.imag], -1).reshape(-1)))
+    assert rel_l2(cplx(host(out), 1, ny, nx)[0], ref) <= 1e-14
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32])
+def test_chain_any_size_matches_oracle(precision):
+    # Gridding -> FFT -> phase on a 120 x 96 grid (mixed radix on both axes)
+    # through rick_plan_image and the separate stages, against the oracle.
+    nx, ny, planes, n = 120, 96, 4, 30000
+    print(f"case chain any-size {nx}x{ny} prec={precision}", flush=True)
+    rng = np.random.default_rng(5)
+    uu = rng.uniform(0.05, 0.95, n)
+    vv = rng.uniform(0.05, 0.75, n)          # v < Ny/Nx: dx = 1/Nx on both axes
+    ww = rng.uniform(0.0, 0.999, n)
+    vr = rng.normal(size=n).astype(np.float32)
+    vi = rng.normal(size=n).astype(np.float32)
+    wt = np.ones(n, np.float32)
+    res, wmin, wmax = 1.0 / (3.0 * nx), -80.0, 120.0
+    ref = O.wstack(O.KAISERBESSEL, planes, uu, vv, ww, vr, vi, wt, 1, 1, 1.0 / nx,
+                   1.0 / planes, 7, nx, ny, threads=8)
+    gs = O.fft_planes(planes, nx, ny, ref, threads=8)
+    ore, oim = O.phase_correction(gs, nx, ny, planes, res, wmin, wmax, threads=8)
+    want = ore + 1j * oim
+    plan = Plan(nx, ny, planes, kernel=O.KAISERBESSEL, support=7, precision=precision,
+                max_points=n, resolution=res, wmin=wmin, wmax=wmax)
+    ins = [dev(a) for a in (uu, vv, ww, vr, vi, wt)]
+    grid = nan_grid(plan)
+    re, im, _ = plan.image(*ins, grid=grid)
+    e_chain = rel_l2(host(re) + 1j * host(im), want)
+    g = plan.grid(*ins)
+    plan.fft(g)
+    re2, im2 = plan.phase(g)
+    e_sep = rel_l2(host(re2) + 1j * host(im2), want)
+    print(f"image rel-L2 vs oracle: chain {e_chain:.3e}, separate {e_sep:.3e}", flush=True)
+    tol = FP64_TOL if precision == FP64 else FP32_TOL
+    assert e_chain <= tol and e_sep <= tol
 
 
 def test_dropin_fft_planes():
