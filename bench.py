@@ -337,9 +337,11 @@ def binms_rates(ds, cfg, torch, path):
 def batch_leg(torch, steps=3):
     """The batched workload: 96 independent imaging jobs (grids 256-2048,
     4-16 w-planes, 5e4-8e6 visibilities, Gaussian support 7 or Kaiser-Bessel
-    support 9), each job's chain captured into one HIP graph per precision,
-    against the same jobs imaged one after another in FP64.  Every job's
-    image is compared with its sequential FP64 image."""
+    support 9), against the same jobs imaged one after another in FP64:
+    each job's chain on four streams captured into one HIP graph, and the
+    fused batch (one launch per stage per group of jobs sharing grid size and
+    kernel) captured likewise.  Every job's image is compared with its
+    sequential FP64 image."""
     from rick_amd import FP32, FP64, batch, synth
     cfgs = batch.job_configs(96)
     datasets = [synth.make_dataset(c, device="cuda:0") for c in cfgs]
@@ -369,7 +371,19 @@ def batch_leg(torch, steps=3):
                      "max_rel_l2_vs_seq_fp64": max(errs)}
         del b
         torch.cuda.empty_cache()
+    # One launch per stage per group of jobs sharing grid size and kernel.
+    for name, prec in (("fused_fp64", FP64), ("fused_fp32", FP32)):
+        fb = batch.FusedBatch(datasets, prec)
+        fb.capture()
+        t_graph = timed(fb.replay, steps, torch) * 1e-3
+        errs = [float(np.linalg.norm(a - r) / np.linalg.norm(r)) for a, r in zip(fb.images(), ref)]
+        out[name] = {"graph_ms": t_graph * 1e3, "groups": len(fb.groups),
+                     "images_per_s": len(cfgs) / t_graph, "vis_per_s": total_vis / t_graph,
+                     "max_rel_l2_vs_seq_fp64": max(errs)}
+        del fb
+        torch.cuda.empty_cache()
     out["fp32_graph_speedup_vs_seq_fp64"] = out["seq_fp64_ms"] / out["fp32"]["graph_ms"]
+    out["fused_fp32_speedup_vs_seq_fp64"] = out["seq_fp64_ms"] / out["fused_fp32"]["graph_ms"]
     del datasets
     torch.cuda.empty_cache()
     return out
@@ -583,8 +597,10 @@ def main():
     if not args.no_batch and not args.only:
         log("batched imaging: 96 jobs, one HIP graph per precision")
         out["batch"] = batch_leg(torch)
-        log(f"batch: fp32 graph {out['batch']['fp32']['graph_ms']:.1f} ms, fp64 graph "
-            f"{out['batch']['fp64']['graph_ms']:.1f} ms, sequential fp64 "
+        log(f"batch: fp32 graph {out['batch']['fp32']['graph_ms']:.1f} ms (fused "
+            f"{out['batch']['fused_fp32']['graph_ms']:.1f} ms), fp64 graph "
+            f"{out['batch']['fp64']['graph_ms']:.1f} ms (fused "
+            f"{out['batch']['fused_fp64']['graph_ms']:.1f} ms), sequential fp64 "
             f"{out['batch']['seq_fp64_ms']:.1f} ms")
     if not args.no_serve and not args.only:
         log(f"serving: {args.serve_seconds:.0f} s of Poisson arrivals per run")

@@ -192,6 +192,42 @@ int rick_plan_image(rick_plan plan, unsigned int num_points, unsigned int freq_p
                     const float *weight, void *grid, double *image_real,
                     double *image_imag, void *stream);
 
+/* Batched imaging: many independent images of one grid size, imaged by one
+ * launch per stage (no counterpart in the reference, which images one grid
+ * per run).  Jobs share the grid size, kernel, support, precision and
+ * channels/polarizations; they differ in rows, w-planes and phase geometry.
+ * The group's w-planes are stacked into one grid; gridding bins every job's
+ * rows in one pass (each block of rows belongs to one job), the tile kernel
+ * runs one task list, the FFT passes run over all planes, and the last
+ * column pass writes each job's image from its own planes.  All pointers of
+ * the jobs are device pointers, fixed at creation: rick_batch_run() reads
+ * their current contents and allocates nothing, so it can be captured in a
+ * HIP graph and replayed. */
+typedef struct {
+    unsigned int num_points;                 /* rows */
+    const double *uu, *vv, *ww;              /* normalised as for wstack()   */
+    const float *vis_real, *vis_img, *weight;
+    int num_w_planes;
+    double dw;                               /* 0: 1/num_w_planes (init.c:12) */
+    double resolution, wmin, wmax;           /* phase_correction.cu:97-102   */
+    double *image_real, *image_imag;         /* Nx*Ny each, overwritten      */
+} rick_batch_job;
+
+typedef struct {
+    int grid_size;                           /* Nx = Ny: power of two, 16..8192 */
+    int kernel, w_support, precision, phase_on;
+    unsigned int freq_per_chan, polarizations;
+    int tile_u, tile_v, rows_per_task, footprint_unroll;   /* 0: defaults */
+    int device;
+} rick_batch_desc;
+
+typedef struct rick_batch_s *rick_batch;
+int rick_batch_create(rick_batch *batch, const rick_batch_desc *desc, const rick_batch_job *jobs,
+                      int njobs);
+int rick_batch_run(rick_batch batch, void *stream);
+int rick_batch_destroy(rick_batch batch);
+size_t rick_batch_device_bytes(rick_batch batch);
+
 /* ------------------------------------------------------------------------- */
 /* 3. Data on either side of the chain: the binMS input format, device-side  */
 /*    NORMALIZE_UVW and sector binning in front of wstack(), and the image   */

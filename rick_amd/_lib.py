@@ -44,6 +44,26 @@ class BinmsMeta(ctypes.Structure):
     ]
 
 
+class BatchJob(ctypes.Structure):
+    """rick_batch_job (include/rick.h)."""
+    _fields_ = [
+        ("num_points", _c_uint), ("uu", _vp), ("vv", _vp), ("ww", _vp), ("vis_real", _vp),
+        ("vis_img", _vp), ("weight", _vp), ("num_w_planes", _c_int), ("dw", _c_double),
+        ("resolution", _c_double), ("wmin", _c_double), ("wmax", _c_double),
+        ("image_real", _vp), ("image_imag", _vp),
+    ]
+
+
+class BatchDesc(ctypes.Structure):
+    """rick_batch_desc (include/rick.h)."""
+    _fields_ = [
+        ("grid_size", _c_int), ("kernel", _c_int), ("w_support", _c_int), ("precision", _c_int),
+        ("phase_on", _c_int), ("freq_per_chan", _c_uint), ("polarizations", _c_uint),
+        ("tile_u", _c_int), ("tile_v", _c_int), ("rows_per_task", _c_int),
+        ("footprint_unroll", _c_int), ("device", _c_int),
+    ]
+
+
 _cp = ctypes.c_char_p
 _up = ctypes.POINTER(_c_uint)
 _ulp = ctypes.POINTER(ctypes.c_ulong)
@@ -76,6 +96,11 @@ SIGNATURES = {
     "rick_plan_grid_stats": (_c_int, [_vp, ctypes.POINTER(_c_uint), _c_int]),
     "rick_plan_image": (_c_int, [_vp, _c_uint, _c_uint, _c_uint, _vp, _vp, _vp, _vp, _vp, _vp,
                                  _vp, _vp, _vp, _vp]),
+    "rick_batch_create": (_c_int, [ctypes.POINTER(_vp), ctypes.POINTER(BatchDesc),
+                                   ctypes.POINTER(BatchJob), _c_int]),
+    "rick_batch_run": (_c_int, [_vp, _vp]),
+    "rick_batch_destroy": (_c_int, [_vp]),
+    "rick_batch_device_bytes": (_c_size, [_vp]),
     "rick_binms_read_meta": (_c_int, [_cp, ctypes.POINTER(BinmsMeta)]),
     "rick_binms_partition": (_c_int, [ctypes.POINTER(BinmsMeta), _c_int, _c_int, _c_uint, _ulp,
                                       _up]),

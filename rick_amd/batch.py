@@ -13,6 +13,7 @@ job, with no host work in between.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import math
 import sys
@@ -135,6 +136,107 @@ class Batch:
         return [(j.re.cpu().numpy() + 1j * j.im.cpu().numpy()) for j in self.jobs]
 
 
+class FusedBatch:
+    """The jobs imaged by one launch per stage for each group of jobs that
+    share grid size, kernel and support (rick_batch_*, include/rick.h): one
+    binning, scan, tile and FFT pass over the group's stacked w-planes, and
+    one fused column/phase pass writing every job's image.  Groups run one
+    after another on one stream (or on a few), and the whole batch is one
+    HIP graph replay after capture()."""
+
+    def __init__(self, datasets, precision, device="cuda:0", n_streams=1, tile=(0, 0),
+                 rows_per_task=0, unroll=0):
+        import torch
+
+        from ._lib import BatchDesc, BatchJob, check, lib
+        self.torch = torch
+        self.datasets = datasets
+        self.precision = precision
+        dev = torch.device(device)
+        self.re, self.im = [], []
+        groups = {}
+        for i, ds in enumerate(datasets):
+            c = ds.cfg
+            groups.setdefault((c.grid, c.kernel, c.support, c.n_chan, c.n_pol), []).append(i)
+        self.groups = []
+        for (grid, kernel, support, F, P), idx in sorted(groups.items()):
+            jobs = (BatchJob * len(idx))()
+            for k, i in enumerate(idx):
+                ds = datasets[i]
+                re = torch.empty(grid * grid, dtype=torch.float64, device=dev)
+                im = torch.empty_like(re)
+                self.re.append((i, re))
+                self.im.append((i, im))
+                j = jobs[k]
+                j.num_points = ds.rows
+                for name in ("uu", "vv", "ww", "vis_real", "vis_img", "weight"):
+                    setattr(j, name, getattr(ds, name).data_ptr())
+                j.num_w_planes = ds.cfg.planes
+                j.dw = ds.dw
+                j.resolution, j.wmin, j.wmax = ds.resolution, ds.wmin, ds.wmax
+                j.image_real, j.image_imag = re.data_ptr(), im.data_ptr()
+            d = BatchDesc()
+            d.grid_size, d.kernel, d.w_support, d.precision = grid, kernel, support, precision
+            d.phase_on = 1
+            d.freq_per_chan, d.polarizations = F, P
+            d.tile_u, d.tile_v = tile
+            d.rows_per_task, d.footprint_unroll = rows_per_task, unroll
+            d.device = dev.index or 0
+            h = ctypes.c_void_p()
+            check(lib().rick_batch_create(ctypes.byref(h), ctypes.byref(d), jobs, len(idx)))
+            self.groups.append((h, jobs, idx))
+        self.re = [t for _, t in sorted(self.re, key=lambda x: x[0])]
+        self.im = [t for _, t in sorted(self.im, key=lambda x: x[0])]
+        self.streams = [torch.cuda.Stream(device=dev) for _ in range(max(1, n_streams))]
+        self.graph = None
+
+    def __del__(self):
+        from . import _lib
+        for h, _, _ in getattr(self, "groups", []):
+            if h and h.value and _lib._lib is not None:
+                _lib._lib.rick_batch_destroy(h)
+        self.groups = []
+
+    @property
+    def device_bytes(self) -> int:
+        from ._lib import lib
+        return sum(lib().rick_batch_device_bytes(h) for h, _, _ in self.groups)
+
+    def _issue(self):
+        from ._lib import check, lib
+        torch = self.torch
+        main = torch.cuda.current_stream()
+        for s in self.streams:
+            s.wait_stream(main)
+        # Largest groups first.
+        order = sorted(range(len(self.groups)),
+                       key=lambda g: -sum(self.datasets[i].cfg.n_vis for i in self.groups[g][2]))
+        for k, gi in enumerate(order):
+            s = self.streams[k % len(self.streams)]
+            check(lib().rick_batch_run(self.groups[gi][0], ctypes.c_void_p(s.cuda_stream)))
+        for s in self.streams:
+            main.wait_stream(s)
+
+    def run_eager(self):
+        self._issue()
+
+    def capture(self):
+        torch = self.torch
+        self.run_eager()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._issue()
+        self.graph = g
+        return g
+
+    def replay(self):
+        self.graph.replay()
+
+    def images(self):
+        return [(r.cpu().numpy() + 1j * i.cpu().numpy()) for r, i in zip(self.re, self.im)]
+
+
 def _timed(fn, steps):
     import torch
     torch.cuda.synchronize()
@@ -179,7 +281,19 @@ def main():
                      "max_rel_l2_vs_fp64": max(errs)}
         del b
         torch.cuda.empty_cache()
+    for name, prec in (("fused_fp64", FP64), ("fused_fp32", FP32)):
+        fb = FusedBatch(datasets, prec)
+        fb.capture()
+        t_graph = _timed(fb.replay, args.steps)
+        imgs = fb.images()
+        errs = [float(np.linalg.norm(a - r) / np.linalg.norm(r)) for a, r in zip(imgs, ref_imgs)]
+        out[name] = {"graph_s": t_graph, "groups": len(fb.groups),
+                     "images_per_s": len(cfgs) / t_graph, "vis_per_s": total_vis / t_graph,
+                     "max_rel_l2_vs_fp64": max(errs)}
+        del fb
+        torch.cuda.empty_cache()
     out["fp32_graph_speedup_vs_seq_fp64"] = t_seq / out["fp32"]["graph_s"]
+    out["fused_fp32_speedup_vs_seq_fp64"] = t_seq / out["fused_fp32"]["graph_s"]
     print(json.dumps(out))
 
 

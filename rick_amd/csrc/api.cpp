@@ -464,6 +464,182 @@ int rick_plan_image(rick_plan p, unsigned int n, unsigned int F, unsigned int P,
     return RICK_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Batched imaging: one plan over the group's stacked w-planes, plus the job
+// tables the gridding and phase kernels read.
+
+}  // extern "C"
+
+struct rick_batch_s {
+    rick_plan plan = nullptr;
+    void *dmem = nullptr;
+    size_t dbytes = 0;
+    BatchJob *jobs_d = nullptr;
+    PhaseJob *pjobs_d = nullptr;
+    void *grid = nullptr;
+    int njobs = 0;
+    uint32_t vrows = 0;
+    unsigned int F = 1, P = 1;
+};
+
+extern "C" {
+
+int rick_batch_create(rick_batch *out, const rick_batch_desc *desc, const rick_batch_job *jobs,
+                      int njobs)
+{
+    clear_error();
+    if (!out || !desc || !jobs) return set_error(RICK_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const rick_batch_desc &d = *desc;
+    const int N = d.grid_size;
+    if (!(is_pow2(N) && N >= 16 && N <= 8192))
+        return set_error(RICK_ERR_UNSUPPORTED,
+                         "batched imaging takes power-of-two grids in [16, 8192], got %d", N);
+    if (njobs < 1) return set_error(RICK_ERR_INVALID, "no jobs");
+    if (d.freq_per_chan == 0 || d.polarizations == 0)
+        return set_error(RICK_ERR_INVALID, "freq_per_chan and polarizations must be >= 1");
+    const unsigned int FP = d.freq_per_chan * d.polarizations;
+    const uint64_t align = batch_row_align();
+    std::vector<BatchJob> bj(njobs);
+    std::vector<PhaseJob> pj(njobs);
+    uint64_t vrows = 0;
+    int planes = 0;
+    for (int j = 0; j < njobs; ++j) {
+        const rick_batch_job &J = jobs[j];
+        if (J.num_w_planes < 1) return set_error(RICK_ERR_INVALID, "job %d: no w-planes", j);
+        if (J.num_points > 0 && (!J.uu || !J.vv || !J.ww || !J.vis_real || !J.vis_img ||
+                                 !J.weight))
+            return set_error(RICK_ERR_INVALID, "job %d: null input array", j);
+        if (!J.image_real || !J.image_imag)
+            return set_error(RICK_ERR_INVALID, "job %d: null image", j);
+        if (FP % 4 == 0 &&
+            ((reinterpret_cast<uintptr_t>(J.vis_real) | reinterpret_cast<uintptr_t>(J.vis_img)) &
+             15u))
+            return set_error(RICK_ERR_INVALID, "job %d: visibility arrays not 16-byte aligned", j);
+        if (d.phase_on && J.num_w_planes > 1 && !(J.wmax != J.wmin))
+            return set_error(RICK_ERR_INVALID, "job %d: wmax == wmin", j);
+        if (!(J.dw >= 0.0)) return set_error(RICK_ERR_INVALID, "job %d: negative dw", j);
+        bj[j] = BatchJob{J.uu, J.vv, J.ww, J.vis_real, J.vis_img, J.weight, J.num_points,
+                         (uint32_t)vrows, planes, J.num_w_planes,
+                         J.dw > 0.0 ? J.dw : 1.0 / J.num_w_planes};
+        // phase_correction.cu:100-102
+        const double dwp = (J.wmax - J.wmin) / (double)J.num_w_planes;
+        pj[j] = PhaseJob{planes, J.num_w_planes, J.wmin, dwp, dwp / (J.wmax - J.wmin),
+                         d.phase_on, nullptr, nullptr, J.image_real, J.image_imag};
+        vrows += ((uint64_t)J.num_points + align - 1) / align * align;
+        planes += J.num_w_planes;
+        if (vrows > (1ull << 30))
+            return set_error(RICK_ERR_INVALID, "batch rows (padded) exceed 2^30");
+    }
+    rick_plan_desc pd{};
+    pd.num_w_planes = planes;
+    pd.grid_size_x = pd.grid_size_y = N;
+    pd.kernel = d.kernel;
+    pd.w_support = d.w_support;
+    pd.dx = 1.0 / N;               // init.c:11
+    pd.dw = 1.0;                   // each job carries its own
+    pd.precision = d.precision;
+    pd.max_points = (unsigned int)std::max<uint64_t>(vrows, 1);
+    pd.resolution = 1.0;
+    pd.wmin = 0.0;
+    pd.wmax = 1.0;
+    pd.phase_on = d.phase_on;
+    pd.tile_u = d.tile_u;
+    pd.tile_v = d.tile_v;
+    pd.rows_per_task = d.rows_per_task;
+    pd.device = d.device;
+    pd.footprint_unroll = d.footprint_unroll;
+    auto b = std::make_unique<rick_batch_s>();
+    int rc = rick_plan_create(&b->plan, &pd);
+    if (rc != RICK_OK) return rc;
+    const size_t csz = d.precision == RICK_FP64 ? 16 : 8;
+    auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+    size_t off = 0;
+    const size_t o_jobs = off; off += al(sizeof(BatchJob) * njobs);
+    const size_t o_pj = off; off += al(sizeof(PhaseJob) * njobs);
+    const size_t o_lm = off; off += al(sizeof(double) * 2 * (size_t)N * njobs);
+    const size_t o_grid = off; off += al(csz * (size_t)planes * N * N);
+    int prev = 0;
+    hipGetDevice(&prev);
+    hipError_t e = hipSetDevice(d.device);
+    if (e == hipSuccess) e = hipMalloc(&b->dmem, off);
+    if (e != hipSuccess) {
+        hipSetDevice(prev);
+        rick_plan_destroy(b->plan);
+        return set_error(RICK_ERR_NOMEM, "batch: hipMalloc(%zu) failed: %s", off,
+                         hipGetErrorString(e));
+    }
+    char *base = static_cast<char *>(b->dmem);
+    b->dbytes = off;
+    b->jobs_d = reinterpret_cast<BatchJob *>(base + o_jobs);
+    b->pjobs_d = reinterpret_cast<PhaseJob *>(base + o_pj);
+    b->grid = base + o_grid;
+    double *lm = reinterpret_cast<double *>(base + o_lm);
+    std::vector<double> tab(2 * (size_t)N);
+    for (int j = 0; j < njobs && e == hipSuccess; ++j) {
+        pixel_sines(N, N, jobs[j].resolution, tab.data());          // phase_correction.cu:204-209
+        pixel_sines(N, N, jobs[j].resolution, tab.data() + N);
+        pj[j].ltab = lm + 2 * (size_t)N * j;
+        pj[j].mtab = lm + 2 * (size_t)N * j + N;
+        e = hipMemcpy(lm + 2 * (size_t)N * j, tab.data(), sizeof(double) * 2 * N,
+                      hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpy(b->jobs_d, bj.data(), sizeof(BatchJob) * njobs, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->pjobs_d, pj.data(), sizeof(PhaseJob) * njobs, hipMemcpyHostToDevice);
+    hipSetDevice(prev);
+    if (e != hipSuccess) {
+        hipFree(b->dmem);
+        rick_plan_destroy(b->plan);
+        return set_error(RICK_ERR_DEVICE, "batch: table upload failed: %s", hipGetErrorString(e));
+    }
+    b->njobs = njobs;
+    b->vrows = (uint32_t)vrows;
+    b->F = d.freq_per_chan;
+    b->P = d.polarizations;
+    *out = b.release();
+    return RICK_OK;
+}
+
+int rick_batch_run(rick_batch b, void *stream)
+{
+    clear_error();
+    if (!b) return set_error(RICK_ERR_INVALID, "null batch");
+    rick_plan p = b->plan;
+    RICK_HIP(hipSetDevice(p->d.device));
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    const bool sparse = p->geom.nx / p->geom.tu <= 256;
+    RICK_HIP(launch_grid(p->geom, p->work, p->d.precision, p->table_d, b->vrows, b->F, b->P,
+                         nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, b->grid, 0, s,
+                         &p->timing, !sparse, b->jobs_d, b->njobs));
+    const double norm = 1.0 / (double)((double)p->geom.nx * (double)p->geom.ny);
+    RICK_HIP(launch_image_chain(p->fft, p->d.precision, b->grid, norm, s, &p->timing,
+                                sparse ? &p->geom : nullptr, sparse ? p->work.counts : nullptr,
+                                sparse ? p->work.band : nullptr, p->phase, nullptr, nullptr,
+                                b->pjobs_d, b->njobs));
+    return RICK_OK;
+}
+
+int rick_batch_destroy(rick_batch b)
+{
+    if (!b) return RICK_OK;
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipSetDevice(b->plan->d.device);
+    if (b->dmem) hipFree(b->dmem);
+    hipSetDevice(dev);
+    rick_plan_destroy(b->plan);
+    delete b;
+    return RICK_OK;
+}
+
+size_t rick_batch_device_bytes(rick_batch b)
+{
+    return b ? b->dbytes + rick_plan_device_bytes(b->plan) : 0;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

@@ -53,6 +53,23 @@ constexpr int kE = 16;  // complex values per thread
 #ifndef RICK_PHASE_RECUR_F64
 #define RICK_PHASE_RECUR_F64 1
 #endif
+// Complex values per thread of the row and column passes.
+#ifndef RICK_ROWS_E_F64
+#define RICK_ROWS_E_F64 16
+#endif
+#ifndef RICK_ROWS_E_F32
+#define RICK_ROWS_E_F32 16
+#endif
+#ifndef RICK_COLS_E_F64
+#define RICK_COLS_E_F64 16
+#endif
+#ifndef RICK_COLS_E_F32
+#define RICK_COLS_E_F32 16
+#endif
+template <typename T>
+constexpr int row_values() { return sizeof(T) == 8 ? RICK_ROWS_E_F64 : RICK_ROWS_E_F32; }
+template <typename T>
+constexpr int col_values() { return sizeof(T) == 8 ? RICK_COLS_E_F64 : RICK_COLS_E_F32; }
 constexpr bool kPhasePrefetchF64 = RICK_PHASE_PREFETCH_F64 != 0;
 constexpr bool kPhaseRecurF64 = RICK_PHASE_RECUR_F64 != 0;
 template <typename T>
@@ -361,13 +378,13 @@ __device__ __forceinline__ void col_rowlive(const ColOcc &co, int plane, int row
 }
 
 // Rows: workgroup w handles rows w*G .. w*G+G-1 of the planes*ny rows.
-template <typename T, int L>
-__global__ void __launch_bounds__(L >= 8192 ? 512 : 256)
+template <typename T, int L, int E>
+__global__ void __launch_bounds__(lines_per_wg(L) * L / E)
 k_fft_rows(int rows_total, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gout, T scale,
            Occupancy oc)
 {
     constexpr int G = lines_per_wg(L);
-    constexpr int NT = G * L / kE;
+    constexpr int NT = G * L / E;
     constexpr int MAXT = 256;  // tiles per row held in LDS (tile width >= 32, L <= 8192)
     // FP64 exchanges split into real and imaginary halves: one T per element
     // in LDS, so twice the workgroups fit per CU (1.61 -> 1.35 ms on 32
@@ -412,10 +429,10 @@ k_fft_rows(int rows_total, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T>
         ln.occ_stride = oc.tiles_u;
         ln.occ_tu = oc.tu;
     }
-    C2<T> vals[kE];
+    C2<T> vals[E];
     GlobalSink<T> sink{gout, &ln, tw, scale};
-    pass<T, L, G, true, 0, GlobalSink<T>, false, kSplit>(vals, threadIdx.x, ln, 1, tw, lds, gin,
-                                                           sink);
+    pass<T, L, G, true, 0, GlobalSink<T>, false, kSplit, E>(vals, threadIdx.x, ln, 1, tw, lds,
+                                                              gin, sink);
 }
 
 // Columns: workgroup (x: strip of G columns, y: row set s, z: plane).
@@ -427,8 +444,8 @@ struct ColArgs {
     int ts;                        // table stride for the L-point DFT
 };
 
-template <typename T, int L, int G>
-__global__ void __launch_bounds__(G * L / kE)
+template <typename T, int L, int G, int E>
+__global__ void __launch_bounds__(G * L / E)
 k_fft_cols(ColArgs a, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gout, T scale,
            ColOcc co)
 {
@@ -448,10 +465,10 @@ k_fft_cols(ColArgs a, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gou
     ln.tw_set = a.tw_by_set ? s : 0;
     ln.occ = nullptr;
     col_rowlive(co, blockIdx.z, s * a.in_set_mul, a.in_stride, rl, ln);
-    C2<T> vals[kE];
+    C2<T> vals[E];
     GlobalSink<T> sink{gout, &ln, tw, scale};
-    pass<T, L, G, false, 0, GlobalSink<T>, false, kSplit>(vals, threadIdx.x, ln, a.ts, tw, lds,
-                                                            gin, sink);
+    pass<T, L, G, false, 0, GlobalSink<T>, false, kSplit, E>(vals, threadIdx.x, ln, a.ts, tw,
+                                                               lds, gin, sink);
 }
 
 // Last column pass fused with the phase correction and plane sum
@@ -498,10 +515,28 @@ struct PhaseSink {
 
 template <typename T, int L, int E>
 __global__ void __launch_bounds__(lines_per_wg(L) * L / E)
-k_fft_cols_phase(ColArgs a, int planes, const C2<T> *__restrict__ tw,
+k_fft_cols_phase(ColArgs a, int planes_in, const C2<T> *__restrict__ tw,
                  const C2<T> *__restrict__ grid, PhaseGeom pg, int vs_mul, int v_stride,
-                 double *__restrict__ ore, double *__restrict__ oim, ColOcc co)
+                 double *ore_in, double *oim_in, ColOcc co, const PhaseJob *__restrict__ pj)
 {
+    // One image (planes 0 .. planes_in-1), or with a batch group's job table
+    // the image of job blockIdx.z from its planes of the group's stack.
+    int planes = planes_in, plane0 = 0;
+    double *__restrict__ ore = ore_in;
+    double *__restrict__ oim = oim_in;
+    if (pj) {
+        const PhaseJob &J = pj[blockIdx.z];
+        planes = J.planes;
+        plane0 = J.plane0;
+        pg.wmin = J.wmin;
+        pg.dw = J.dw;
+        pg.dwnorm = J.dwnorm;
+        pg.phase_on = J.phase_on;
+        pg.ltab = J.ltab;
+        pg.mtab = J.mtab;
+        ore = J.re;
+        oim = J.im;
+    }
     constexpr int G = lines_per_wg(L);
     constexpr int NT = G * L / E;
     __shared__ C2<T> lds[lds_elems(L, false)];
@@ -578,8 +613,9 @@ k_fft_cols_phase(ColArgs a, int planes, const C2<T> *__restrict__ tw,
         }
     }
     auto plane_lines = [&](int iw) {
-        if (co.band) col_rowlive(co, iw, s * a.in_set_mul, a.in_stride, rl[iw & 1], ln);
-        ln.in_base = (size_t)iw * (size_t)a.nx * (size_t)a.ny +
+        const int pw = plane0 + iw;   // plane in the grid's (group's) stack
+        if (co.band) col_rowlive(co, pw, s * a.in_set_mul, a.in_stride, rl[iw & 1], ln);
+        ln.in_base = (size_t)pw * (size_t)a.nx * (size_t)a.ny +
                      (size_t)s * a.in_set_mul * (size_t)a.nx + c0;
     };
     // FP64 runs one wave per SIMD at this register count: the next plane's
@@ -645,8 +681,9 @@ hipError_t launch_rows(int L, int rows_total, const C2<T> *tw, const C2<T> *in, 
 #define RICK_ROWS(LL)                                                                      \
     case LL: {                                                                             \
         constexpr int G = lines_per_wg(LL);                                                \
-        hipLaunchKernelGGL((k_fft_rows<T, LL>), dim3((rows_total + G - 1) / G),            \
-                           dim3(G * LL / kE), 0, s, rows_total, tw, in, out, scale, oc);   \
+        constexpr int E = row_values<T>();                                                 \
+        hipLaunchKernelGGL((k_fft_rows<T, LL, E>), dim3((rows_total + G - 1) / G),         \
+                           dim3(G * LL / E), 0, s, rows_total, tw, in, out, scale, oc);    \
         return hipGetLastError();                                                          \
     }
     switch (L) {
@@ -664,8 +701,9 @@ hipError_t launch_cols(int L, const ColArgs &a, int sets, int planes, const C2<T
 #define RICK_COLS(LL)                                                                        \
     case LL: {                                                                               \
         constexpr int G = col_lines<T>(LL);                                                  \
-        hipLaunchKernelGGL((k_fft_cols<T, LL, G>), dim3((a.nx + G - 1) / G, sets, planes),   \
-                           dim3(G * LL / kE), 0, s, a, tw, in, out, T(1), co);               \
+        constexpr int E = col_values<T>();                                                   \
+        hipLaunchKernelGGL((k_fft_cols<T, LL, G, E>), dim3((a.nx + G - 1) / G, sets, planes), \
+                           dim3(G * LL / E), 0, s, a, tw, in, out, T(1), co);                \
         return hipGetLastError();                                                            \
     }
     switch (L) {
@@ -679,15 +717,16 @@ hipError_t launch_cols(int L, const ColArgs &a, int sets, int planes, const C2<T
 template <typename T>
 hipError_t launch_cols_phase(int L, const ColArgs &a, int sets, int planes, const C2<T> *tw,
                              const C2<T> *grid, const PhaseGeom &pg, int vs_mul, int v_stride,
-                             double *re, double *im, hipStream_t s, const ColOcc &co)
+                             double *re, double *im, hipStream_t s, const ColOcc &co,
+                             const PhaseJob *pj, int njobs)
 {
 #define RICK_CPH(LL)                                                                         \
     case LL: {                                                                               \
         constexpr int G = lines_per_wg(LL);                                                  \
         constexpr int E = phase_values<T>(LL);                                               \
-        hipLaunchKernelGGL((k_fft_cols_phase<T, LL, E>), dim3((a.nx + G - 1) / G, sets, 1),  \
-                           dim3(G * LL / E), 0, s, a, planes, tw, grid, pg, vs_mul,          \
-                           v_stride, re, im, co);                                            \
+        hipLaunchKernelGGL((k_fft_cols_phase<T, LL, E>),                                     \
+                           dim3((a.nx + G - 1) / G, sets, pj ? njobs : 1), dim3(G * LL / E), \
+                           0, s, a, planes, tw, grid, pg, vs_mul, v_stride, re, im, co, pj); \
         return hipGetLastError();                                                            \
     }
     switch (L) {
@@ -762,7 +801,7 @@ hipError_t fft_impl(const FftPlan &p, const void *in_v, void *out_v, void *tmp_v
 template <typename T>
 hipError_t image_chain_impl(const FftPlan &p, void *grid_v, double scale_d, hipStream_t s,
                             Timing *tm, const Occupancy &oc, const PhaseGeom &pg, double *re,
-                            double *im)
+                            double *im, const PhaseJob *pj, int njobs)
 {
     using CT = C2<T>;
     CT *grid = static_cast<CT *>(grid_v);
@@ -784,7 +823,8 @@ hipError_t image_chain_impl(const FftPlan &p, void *grid_v, double scale_d, hipS
     if (!fft_needs_tmp(sizeof(T) == 8 ? RICK_FP64 : RICK_FP32, ny)) {
         const ColArgs a{nx, ny, 0, 1, 0, 1, 0, 1};
         tm->begin(RICK_STAGE_PHASE, s);
-        e = launch_cols_phase<T>(ny, a, 1, planes, twy, grid, pg, 0, 1, re, im, s, co);
+        e = launch_cols_phase<T>(ny, a, 1, planes, twy, grid, pg, 0, 1, re, im, s, co, pj,
+                                 njobs);
         tm->end(RICK_STAGE_PHASE, s);
         return e;
     }
@@ -798,7 +838,8 @@ hipError_t image_chain_impl(const FftPlan &p, void *grid_v, double scale_d, hipS
     tm->end(RICK_STAGE_FFT_COLS, s);
     if (e != hipSuccess) return e;
     tm->begin(RICK_STAGE_PHASE, s);
-    e = launch_cols_phase<T>(N2, Bt, N1, planes, twy, grid, pg, 1, N1, re, im, s, ColOcc{});
+    e = launch_cols_phase<T>(N2, Bt, N1, planes, twy, grid, pg, 1, N1, re, im, s, ColOcc{}, pj,
+                             njobs);
     tm->end(RICK_STAGE_PHASE, s);
     return e;
 }
@@ -823,7 +864,7 @@ hipError_t launch_fft(const FftPlan &p, int precision, const void *in, void *out
 hipError_t launch_image_chain(const FftPlan &p, int precision, void *grid, double scale,
                               hipStream_t s, Timing *tm, const GridGeom *occ_geom,
                               const uint32_t *occ_counts, uint32_t *occ_band, const PhaseGeom &pg,
-                              double *re, double *im)
+                              double *re, double *im, const PhaseJob *jobs, int njobs)
 {
     Timing off;
     if (tm == nullptr) tm = &off;
@@ -839,8 +880,8 @@ hipError_t launch_image_chain(const FftPlan &p, int precision, void *grid, doubl
         oc.ny = occ_geom->ny;
     }
     if (precision == RICK_FP64)
-        return image_chain_impl<double>(p, grid, scale, s, tm, oc, pg, re, im);
-    return image_chain_impl<float>(p, grid, scale, s, tm, oc, pg, re, im);
+        return image_chain_impl<double>(p, grid, scale, s, tm, oc, pg, re, im, jobs, njobs);
+    return image_chain_impl<float>(p, grid, scale, s, tm, oc, pg, re, im, jobs, njobs);
 }
 
 }  // namespace rick

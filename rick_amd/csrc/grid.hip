@@ -61,6 +61,7 @@ constexpr uint32_t kDirect = 0x80000000u;     // item flag: key without a hash s
 constexpr int kBinThreads = 1024;
 constexpr int kBinRowsPerThread = 16;
 constexpr int kBinRows = kBinThreads * kBinRowsPerThread;
+static_assert(kBinRows % kRowsPerBlock == 0, "a scatter block must not span two batch jobs");
 constexpr int kBinHashBits = 13;
 constexpr int kBinSlots = 1 << kBinHashBits;
 constexpr int kMaxProbe = 32;
@@ -101,14 +102,17 @@ __device__ __forceinline__ bool in_int_range(double x)
 
 // Footprint of a row whose positions (in cells) are pu, pv and w-plane
 // coordinate wi; w-stacking.cu:464-476 with the clip made explicit.
+// A batch job's rows check their plane against the job's `planes` and land
+// at plane0 + gw of the group's plane stack.
 __device__ __forceinline__ bool footprint(const GridGeom &g, double pu, double pv, double wi,
-                                          Foot &f)
+                                          Foot &f, int planes, int plane0)
 {
     if (!in_int_range(wi) || !in_int_range(pu) || !in_int_range(pv)) return false;
     f.gw = (int)wi;
     f.gu = (int)pu;
     f.gv = (int)pv;
-    if (f.gw < 0 || f.gw >= g.planes) return false;
+    if (f.gw < 0 || f.gw >= planes) return false;
+    f.gw += plane0;
     f.jmin = (f.gu > g.kl - 1) ? f.gu - g.kl : 0;
     f.jmax = (f.gu < g.nx - g.kl) ? f.gu + g.kl : g.nx - 1;
     f.kmin = (f.gv > g.kl - 1) ? f.gv - g.kl : 0;
@@ -125,6 +129,37 @@ __device__ __forceinline__ void uv_range(const GridGeom &g, double pu, double pv
     f.jmax = (f.gu < g.nx - g.kl) ? f.gu + g.kl : g.nx - 1;
     f.kmin = (f.gv > g.kl - 1) ? f.gv - g.kl : 0;
     f.kmax = (f.gv < g.ny - g.kl) ? f.gv + g.kl : g.ny - 1;
+}
+
+// The rows a gridding block reads: its job's arrays, row count, w-plane
+// spacing and planes, and the block's first row in them.
+struct BlockRows {
+    const double *uu, *vv, *ww;
+    const float *vr, *vi, *wt;
+    uint32_t n, base;
+    double dw;
+    int planes, plane0;
+};
+
+// Block whose first virtual row is vrow: a single job (jobs == null), or
+// the batch job holding it (jobs sorted by row0, row0 a multiple of the
+// block's rows, so a block never spans two jobs).
+__device__ __forceinline__ BlockRows block_rows(const GridGeom &g, uint32_t vrow, uint32_t n,
+                                                const double *uu, const double *vv,
+                                                const double *ww, const float *vr,
+                                                const float *vi, const float *wt,
+                                                const BatchJob *jobs, int njobs)
+{
+    if (jobs == nullptr) return BlockRows{uu, vv, ww, vr, vi, wt, n, vrow, g.dw, g.planes, 0};
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].row0 <= vrow) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchJob &J = jobs[lo];
+    return BlockRows{J.uu, J.vv, J.ww, J.vr, J.vi, J.wt, J.n, vrow - J.row0, J.dw, J.planes,
+                     J.plane0};
 }
 
 // Buckets of a footprint (support <= tile size, so at most 2 tiles per axis).
@@ -180,9 +215,11 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v)
 
 // 8 waves per SIMD: two 1024-thread blocks per CU (their hash tables take
 // 64 KB of LDS each), which needs the SGPR count held to the 8-wave budget.
+template <bool BATCH>
 __global__ void __launch_bounds__(kBinThreads, 8)
-k_count(GridGeom g, uint32_t n, const double *__restrict__ uu, const double *__restrict__ vv,
-        const double *__restrict__ ww, uint32_t *__restrict__ counts)
+k_count(GridGeom g, uint32_t n_all, const double *__restrict__ uu_all,
+        const double *__restrict__ vv_all, const double *__restrict__ ww_all,
+        uint32_t *__restrict__ counts, const BatchJob *__restrict__ jobs, int njobs)
 {
     __shared__ uint32_t hkey[kBinSlots], hcnt[kBinSlots];
     for (int i = threadIdx.x; i < kBinSlots; i += kBinThreads) {
@@ -190,7 +227,13 @@ k_count(GridGeom g, uint32_t n, const double *__restrict__ uu, const double *__r
         hcnt[i] = 0;
     }
     __syncthreads();
-    const uint32_t base = blockIdx.x * (uint32_t)kBinRows;
+    const BlockRows br = block_rows(g, blockIdx.x * (uint32_t)kBinRows, n_all, uu_all, vv_all,
+                                    ww_all, nullptr, nullptr, nullptr, BATCH ? jobs : nullptr,
+                                    njobs);
+    const double *__restrict__ uu = br.uu;
+    const double *__restrict__ vv = br.vv;
+    const double *__restrict__ ww = br.ww;
+    const uint32_t n = br.n, base = br.base;
     // The next row's coordinates are loaded before the current row is hashed,
     // so their latency overlaps the LDS work.
     uint32_t i = base + threadIdx.x;
@@ -209,7 +252,7 @@ k_count(GridGeom g, uint32_t n, const double *__restrict__ uu, const double *__r
             wn = ww[in];
         }
         Foot f;
-        if (i < n && footprint(g, u / g.dx, v / g.dx, w / g.dw, f)) {
+        if (i < n && footprint(g, u / g.dx, v / g.dx, w / br.dw, f, br.planes, br.plane0)) {
             uint32_t keys[4];
             const int nk = row_keys(g, f, keys);
             for (int q = 0; q < nk; ++q) {
@@ -483,14 +526,24 @@ __device__ __forceinline__ void wave_row_sums(uint32_t r0, uint32_t n, uint32_t 
 // VMODE: 0 = per-row scalar reads, 1 = per-row 16-byte reads (F*P a multiple
 // of 4), 2 or 4 = wave-cooperative reads with VMODE lanes per row (F*P = 8 or
 // 16); 16-byte modes need 16-byte aligned visibility arrays.
-template <int PREC, int VMODE>
+template <int PREC, int VMODE, bool BATCH>
 __global__ void __launch_bounds__(kScatterThreads, 8)
-k_scatter(GridGeom g, uint32_t n, uint32_t F, uint32_t P, const double *__restrict__ uu,
-          const double *__restrict__ vv, const double *__restrict__ ww,
-          const float *__restrict__ vr, const float *__restrict__ vi,
-          const float *__restrict__ wt, uint32_t *__restrict__ cursor,
-          typename RecOf<PREC>::type *__restrict__ rec, uint32_t *__restrict__ meta)
+k_scatter(GridGeom g, uint32_t n_all, uint32_t F, uint32_t P, const double *__restrict__ uu_all,
+          const double *__restrict__ vv_all, const double *__restrict__ ww_all,
+          const float *__restrict__ vr_all, const float *__restrict__ vi_all,
+          const float *__restrict__ wt_all, uint32_t *__restrict__ cursor,
+          typename RecOf<PREC>::type *__restrict__ rec, uint32_t *__restrict__ meta,
+          const BatchJob *__restrict__ jobs, int njobs)
 {
+    const BlockRows br = block_rows(g, blockIdx.x * kRowsPerBlock, n_all, uu_all, vv_all, ww_all,
+                                    vr_all, vi_all, wt_all, BATCH ? jobs : nullptr, njobs);
+    const double *__restrict__ uu = br.uu;
+    const double *__restrict__ vv = br.vv;
+    const double *__restrict__ ww = br.ww;
+    const float *__restrict__ vr = br.vr;
+    const float *__restrict__ vi = br.vi;
+    const float *__restrict__ wt = br.wt;
+    const uint32_t n = br.n;
     __shared__ uint32_t hkey[kHashSlots], hcnt[kHashSlots], hbase[kHashSlots];
     __shared__ unsigned long long smax[kScatterThreads / 64];
     for (int i = threadIdx.x; i < kHashSlots; i += kScatterThreads) {
@@ -498,7 +551,7 @@ k_scatter(GridGeom g, uint32_t n, uint32_t F, uint32_t P, const double *__restri
         hcnt[i] = 0;
     }
     __syncthreads();
-    const uint32_t base = blockIdx.x * kRowsPerBlock;
+    const uint32_t base = br.base;
     // The read path is chosen on the host, so each instantiation carries one.
     constexpr bool vec4 = VMODE == 1;
     double pu[kRowsPerThread], pv[kRowsPerThread], sre[kRowsPerThread], sim[kRowsPerThread];
@@ -516,7 +569,7 @@ k_scatter(GridGeom g, uint32_t n, uint32_t F, uint32_t P, const double *__restri
         if (i >= n) continue;
         pu[k] = uu[i] / g.dx;
         pv[k] = vv[i] / g.dx;
-        if (!footprint(g, pu[k], pv[k], ww[i] / g.dw, f)) continue;
+        if (!footprint(g, pu[k], pv[k], ww[i] / br.dw, f, br.planes, br.plane0)) continue;
         // Channel/polarization sum, w-stacking.cu:512-527 (weights indexed
         // row*P + p, shared across channels; float product first).
         const uint32_t FP = F * P;
@@ -941,25 +994,28 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, uint32_t *__restrict__ meta,
 
         if (PREC == RICK_FP32 && det) {
             unsigned long long *acc = split_acc + (size_t)slot * 2 * (TU * TV);
+            unsigned long long done = 0ull;
             for (int idx = threadIdx.x; idx < nk * TU; idx += kTileThreads) {
                 const int kk = idx / TU, jj = idx % TU;
                 if (jj >= nj) continue;
                 const int li = kk * LDU + jj;
                 const unsigned long long a = reinterpret_cast<const unsigned long long *>(sre)[li];
                 const unsigned long long b2 = reinterpret_cast<const unsigned long long *>(sim)[li];
-                if (a) atomicAdd(&acc[idx], a);
-                if (b2) atomicAdd(&acc[TU * TV + idx], b2);
+                // Returning atomics: each has been performed (at the device's
+                // coherence point) once its old value is back, so all of them
+                // are in before the ticket below is taken -- no cache flush.
+                if (a) done ^= atomicAdd(&acc[idx], a);
+                if (b2) done ^= atomicAdd(&acc[TU * TV + idx], b2);
             }
             // The bucket's last task to get here writes the tile and clears
             // the accumulator and ticket for the next call.
-            __threadfence();
+            if (done == 0x5a5a5a5a5a5a5a5aull) last_sh = 2u;   // keeps the old values live
             __syncthreads();
             if (threadIdx.x == 0)
                 last_sh = atomicAdd(&split_ticket[slot], 1u) ==
                           ntasks_of(offsets[b + 1] - offsets[b], g.rmax) - 1u;
             __syncthreads();
             if (last_sh) {
-                __threadfence();
                 const double inv = 1.0 / bucket_scale;   // exact: a power of two
                 T2 *g2 = reinterpret_cast<T2 *>(grid);
                 for (int idx = threadIdx.x; idx < nk * TU; idx += kTileThreads) {
@@ -1115,6 +1171,8 @@ void capacities(const GridGeom &g, uint32_t max_points, uint32_t *rec_cap, uint3
 
 }  // namespace
 
+uint32_t batch_row_align() { return (uint32_t)kBinRows; }
+
 const char *grid_launch_limit(const GridGeom &g, uint32_t max_points)
 {
     uint32_t rec_cap, split_cap, task_cap, nscan;
@@ -1191,7 +1249,8 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
                        const double *table_d, uint32_t n, uint32_t F, uint32_t P,
                        const double *uu, const double *vv, const double *ww, const float *vr,
                        const float *vi, const float *wt, void *grid, int accumulate,
-                       hipStream_t s, Timing *tm, bool clear_empty)
+                       hipStream_t s, Timing *tm, bool clear_empty, const BatchJob *jobs,
+                       int njobs)
 {
     Timing off;
     if (tm == nullptr) tm = &off;
@@ -1202,9 +1261,14 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
     const int zero = (!accumulate && clear_empty) ? 1 : 0;
     tm->begin(RICK_STAGE_COUNT_ROWS, s);
     hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks), dim3(256), 0, s, w.counts, g.nb, 0u);
-    if (n > 0)
-        hipLaunchKernelGGL(k_count, dim3(cblk), dim3(kBinThreads), 0, s, g, n, uu, vv, ww,
-                           w.counts);
+    if (n > 0) {
+        if (jobs)
+            hipLaunchKernelGGL(k_count<true>, dim3(cblk), dim3(kBinThreads), 0, s, g, n, uu, vv,
+                               ww, w.counts, jobs, njobs);
+        else
+            hipLaunchKernelGGL(k_count<false>, dim3(cblk), dim3(kBinThreads), 0, s, g, n, uu, vv,
+                               ww, w.counts, jobs, njobs);
+    }
     tm->end(RICK_STAGE_COUNT_ROWS, s);
     tm->begin(RICK_STAGE_SCAN, s);
     hipLaunchKernelGGL(k_scan_reduce, dim3(w.nscan), dim3(kThreads), 0, s, g, w.counts,
@@ -1220,12 +1284,19 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
         const uint32_t FP = F * P;
         const bool vec4 = (FP % 4u) == 0u &&
                           ((reinterpret_cast<uintptr_t>(vr) | reinterpret_cast<uintptr_t>(vi)) &
-                           15u) == 0u;
+                           15u) == 0u;   // batch: vr = vi = null, the caller checked the jobs
         const int mode = !vec4 ? 0 : FP == 8u ? 2 : FP == 16u ? 4 : 1;
 #define RICK_SCATTER(PR, VM)                                                                  \
-    hipLaunchKernelGGL((k_scatter<PR, VM>), dim3(nblk), dim3(kScatterThreads), 0, s, g, n, F, P, \
-                       uu, vv, ww, vr, vi, wt, w.cursor,                                      \
-                       static_cast<typename RecOf<PR>::type *>(w.records), w.meta)
+    if (jobs)                                                                                 \
+        hipLaunchKernelGGL((k_scatter<PR, VM, true>), dim3(nblk), dim3(kScatterThreads), 0, s, \
+                           g, n, F, P, uu, vv, ww, vr, vi, wt, w.cursor,                      \
+                           static_cast<typename RecOf<PR>::type *>(w.records), w.meta, jobs,  \
+                           njobs);                                                            \
+    else                                                                                      \
+        hipLaunchKernelGGL((k_scatter<PR, VM, false>), dim3(nblk), dim3(kScatterThreads), 0,  \
+                           s, g, n, F, P, uu, vv, ww, vr, vi, wt, w.cursor,                   \
+                           static_cast<typename RecOf<PR>::type *>(w.records), w.meta, jobs,  \
+                           njobs)
 #define RICK_SCATTER_MODES(PR)                                                                \
     switch (mode) {                                                                           \
     case 4: RICK_SCATTER(PR, 4); break;                                                       \

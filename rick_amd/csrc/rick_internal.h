@@ -93,6 +93,22 @@ struct GridWork {
     uint32_t rec_cap, split_cap, task_cap, nscan;
 };
 
+// One job of a batch group (jobs sharing grid size, kernel, support, tile
+// shape and channels/polarizations): its rows, w-plane spacing and the
+// group's w-planes it owns.  Rows are numbered in a virtual row space in
+// which job j starts at row0 (a multiple of the count kernel's block rows),
+// so every gridding block belongs to one job.
+struct BatchJob {
+    const double *uu, *vv, *ww;
+    const float *vr, *vi, *wt;
+    uint32_t n;             // rows
+    uint32_t row0;          // first virtual row
+    int plane0, planes;     // the job's planes in the group's plane stack
+    double dw;
+};
+// Rows per block of the count kernel: batch jobs start at multiples of it.
+uint32_t batch_row_align();
+
 size_t grid_work_bytes(const GridGeom &g, uint32_t max_points, int precision);
 // Null if every gridding launch of a plan with this geometry and capacity
 // fits HIP's launch limits, else why not.
@@ -101,11 +117,16 @@ void grid_work_carve(const GridGeom &g, uint32_t max_points, int precision, void
                      GridWork *w);
 
 // Launches the binned gridder.  Returns a hipError_t.
+// With jobs (device array of njobs BatchJob), num_points is the size of the
+// virtual row space and the input pointers are the jobs' own; the scatter's
+// read mode then assumes 16-byte aligned visibility arrays when F*P is a
+// multiple of 4 (checked by the caller).
 hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
                        const double *table_d, uint32_t num_points, uint32_t F, uint32_t P,
                        const double *uu, const double *vv, const double *ww,
                        const float *vr, const float *vi, const float *wt, void *grid,
-                       int accumulate, hipStream_t s, Timing *tm, bool clear_empty = true);
+                       int accumulate, hipStream_t s, Timing *tm, bool clear_empty = true,
+                       const BatchJob *jobs = nullptr, int njobs = 0);
 
 // FFT: per-plane 2-D backward transform of planes x ny x nx complex, scaled.
 struct FftPlan {
@@ -165,9 +186,19 @@ hipError_t launch_phase(const PhaseGeom &p, int precision, const void *gridss, d
 // rows of tile rows without any (flags written to occ_band, planes*tiles_v),
 // column FFT whose last pass is fused with the phase correction and plane sum
 // into the images (overwritten).  Scales by `scale` (1/(Nx*Ny)).
+// Phase geometry and output of one job of a batch group (its planes in the
+// group's plane stack and its images).
+struct PhaseJob {
+    int plane0, planes;
+    double wmin, dw, dwnorm;
+    int phase_on;
+    const double *ltab, *mtab;
+    double *re, *im;
+};
 hipError_t launch_image_chain(const FftPlan &p, int precision, void *grid, double scale,
                               hipStream_t s, Timing *tm, const GridGeom *occ_geom,
                               const uint32_t *occ_counts, uint32_t *occ_band, const PhaseGeom &pg,
-                              double *re, double *im);
+                              double *re, double *im, const PhaseJob *jobs = nullptr,
+                              int njobs = 0);
 
 }  // namespace rick

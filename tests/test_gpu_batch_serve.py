@@ -21,7 +21,7 @@ import torch
 
 from conftest import rel_l2
 from oracle import oracle as O
-from rick_amd import FP32, FP64, batch, serve, synth
+from rick_amd import FP32, FP64, batch, plan_for, serve, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -130,18 +130,76 @@ def test_batch_96_jobs_against_oracle():
         imgs[prec] = b.images()
         del b
         torch.cuda.empty_cache()
+    # The same jobs through the fused batch (one launch per stage per group of
+    # jobs sharing grid size and kernel), captured and replayed.
+    fused = {}
+    for prec in (FP64, FP32):
+        fb = batch.FusedBatch(datasets, prec)
+        fb.capture()
+        for rep in range(2):
+            for t in fb.re + fb.im:
+                t.fill_(float("nan"))
+            fb.replay()
+            torch.cuda.synchronize()
+        fused[prec] = fb.images()
+        del fb
+        torch.cuda.empty_cache()
     worst64 = worst32 = 0.0
     for (grid, kernel), i in sorted(picks.items()):
         ref = oracle_image(datasets[i])
         e64, e32 = rel_l2(imgs[FP64][i], ref), rel_l2(imgs[FP32][i], ref)
+        f64, f32 = rel_l2(fused[FP64][i], ref), rel_l2(fused[FP32][i], ref)
         print(f"job {i} ({grid}^2 x {cfgs[i].planes}, kernel {kernel}, {cfgs[i].n_vis} vis): "
-              f"FP64 {e64:.2e}, FP32 {e32:.2e} vs oracle", flush=True)
-        worst64, worst32 = max(worst64, e64), max(worst32, e32)
+              f"FP64 {e64:.2e}, FP32 {e32:.2e}, fused FP64 {f64:.2e}, fused FP32 {f32:.2e} "
+              f"vs oracle", flush=True)
+        worst64 = max(worst64, e64, f64)
+        worst32 = max(worst32, e32, f32)
     e = max(rel_l2(a, b) for a, b in zip(imgs[FP32], imgs[FP64]))
+    ef64 = max(rel_l2(a, b) for a, b in zip(fused[FP64], imgs[FP64]))
+    ef32 = max(rel_l2(a, b) for a, b in zip(fused[FP32], imgs[FP64]))
     print(f"oracle picks: FP64 max {worst64:.2e}, FP32 max {worst32:.2e}; all 96 FP32 vs FP64 "
-          f"max {e:.2e}", flush=True)
+          f"max {e:.2e}; fused FP64 vs per-job FP64 max {ef64:.2e}, fused FP32 vs FP64 max "
+          f"{ef32:.2e}", flush=True)
     assert len(picks) == 8
     assert worst64 <= 1e-12 and worst32 <= 1e-5 and e <= 1e-5
+    assert ef64 <= 1e-12 and ef32 <= 1e-5
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32])
+def test_fused_batch_edge_jobs(precision):
+    # Jobs of one group with no rows, rows not a multiple of the binning
+    # block, one plane, NaN rows, and two groups; each image against the job
+    # imaged alone by its own plan.
+    cfgs = small_cfgs(6, seed=5)
+    cfgs = [c.scaled(grid=256) for c in cfgs[:4]] + [c.scaled(grid=512) for c in cfgs[4:]]
+    cfgs[1] = cfgs[1].scaled(planes=1)
+    datasets = [synth.make_dataset(c, device="cuda:0") for c in cfgs]
+    datasets[2] = datasets[2].subset(slice(0, 0))               # no rows
+    datasets[3] = datasets[3].subset(slice(0, 16384 + 3))       # past one block
+    bad = datasets[0].subset(slice(0, datasets[0].rows))
+    uu = bad.uu.clone()
+    uu[::7] = float("nan")                                       # dropped rows
+    bad.uu = uu
+    datasets[0] = bad
+    print(f"case fused batch edge jobs prec={precision}: rows {[d.rows for d in datasets]}",
+          flush=True)
+    fb = batch.FusedBatch(datasets, precision)
+    assert len(fb.groups) == len({(c.grid, c.kernel) for c in cfgs})
+    fb.run_eager()
+    torch.cuda.synchronize()
+    got = fb.images()
+    for k, ds in enumerate(datasets):
+        if ds.rows == 0:
+            assert np.count_nonzero(got[k]) == 0
+            continue
+        plan = plan_for(ds, precision=precision)
+        re, im, _ = plan.image(ds.uu, ds.vv, ds.ww, ds.vis_real, ds.vis_img, ds.weight,
+                               ds.cfg.n_chan, ds.cfg.n_pol)
+        ref = re.cpu().numpy() + 1j * im.cpu().numpy()
+        err = rel_l2(got[k], ref)
+        print(f"job {k}: fused vs alone {err:.2e}", flush=True)
+        assert np.isfinite(got[k]).all()
+        assert err <= (1e-12 if precision == FP64 else 1e-6)
 
 
 def test_serve_lofar_mix_bitwise_and_oracle():
