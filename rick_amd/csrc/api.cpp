@@ -181,6 +181,8 @@ int rick_plan_create(rick_plan *out, const rick_plan_desc *desc)
     g.kind = d.kernel;
     g.dx = d.dx;
     g.dw = d.dw;
+    g.inv_dx = exact_inverse(d.dx);
+    g.inv_dw = exact_inverse(d.dw);
     g.tu = tu;
     g.tv = tv;
     g.tiles_u = (g.nx + tu - 1) / tu;
@@ -520,9 +522,9 @@ int rick_batch_create(rick_batch *out, const rick_batch_desc *desc, const rick_b
         if (d.phase_on && J.num_w_planes > 1 && !(J.wmax != J.wmin))
             return set_error(RICK_ERR_INVALID, "job %d: wmax == wmin", j);
         if (!(J.dw >= 0.0)) return set_error(RICK_ERR_INVALID, "job %d: negative dw", j);
+        const double jdw = J.dw > 0.0 ? J.dw : 1.0 / J.num_w_planes;   // init.c:12
         bj[j] = BatchJob{J.uu, J.vv, J.ww, J.vis_real, J.vis_img, J.weight, J.num_points,
-                         (uint32_t)vrows, planes, J.num_w_planes,
-                         J.dw > 0.0 ? J.dw : 1.0 / J.num_w_planes};
+                         (uint32_t)vrows, planes, J.num_w_planes, jdw, exact_inverse(jdw)};
         // phase_correction.cu:100-102
         const double dwp = (J.wmax - J.wmin) / (double)J.num_w_planes;
         pj[j] = PhaseJob{planes, J.num_w_planes, J.wmin, dwp, dwp / (J.wmax - J.wmin),

@@ -131,13 +131,22 @@ __device__ __forceinline__ void uv_range(const GridGeom &g, double pu, double pv
     f.kmax = (f.gv < g.ny - g.kl) ? f.gv + g.kl : g.ny - 1;
 }
 
+// x / d, the reference's IEEE division (w-stacking.cu:464-466).  When d is a
+// power of two with an exact reciprocal inv (inv != 0), x * inv is that same
+// correctly rounded quotient (both are the exact value x * 2^-k, rounded
+// once), at the cost of one multiply instead of a division sequence.
+__device__ __forceinline__ double div_by(double x, double d, double inv)
+{
+    return inv != 0.0 ? x * inv : x / d;
+}
+
 // The rows a gridding block reads: its job's arrays, row count, w-plane
 // spacing and planes, and the block's first row in them.
 struct BlockRows {
     const double *uu, *vv, *ww;
     const float *vr, *vi, *wt;
     uint32_t n, base;
-    double dw;
+    double dw, inv_dw;
     int planes, plane0;
 };
 
@@ -150,7 +159,8 @@ __device__ __forceinline__ BlockRows block_rows(const GridGeom &g, uint32_t vrow
                                                 const float *vi, const float *wt,
                                                 const BatchJob *jobs, int njobs)
 {
-    if (jobs == nullptr) return BlockRows{uu, vv, ww, vr, vi, wt, n, vrow, g.dw, g.planes, 0};
+    if (jobs == nullptr)
+        return BlockRows{uu, vv, ww, vr, vi, wt, n, vrow, g.dw, g.inv_dw, g.planes, 0};
     int lo = 0, hi = njobs - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -158,8 +168,8 @@ __device__ __forceinline__ BlockRows block_rows(const GridGeom &g, uint32_t vrow
         else hi = mid - 1;
     }
     const BatchJob &J = jobs[lo];
-    return BlockRows{J.uu, J.vv, J.ww, J.vr, J.vi, J.wt, J.n, vrow - J.row0, J.dw, J.planes,
-                     J.plane0};
+    return BlockRows{J.uu, J.vv, J.ww, J.vr, J.vi, J.wt, J.n, vrow - J.row0, J.dw, J.inv_dw,
+                     J.planes, J.plane0};
 }
 
 // Buckets of a footprint (support <= tile size, so at most 2 tiles per axis).
@@ -252,7 +262,8 @@ k_count(GridGeom g, uint32_t n_all, const double *__restrict__ uu_all,
             wn = ww[in];
         }
         Foot f;
-        if (i < n && footprint(g, u / g.dx, v / g.dx, w / br.dw, f, br.planes, br.plane0)) {
+        if (i < n && footprint(g, div_by(u, g.dx, g.inv_dx), div_by(v, g.dx, g.inv_dx),
+                               div_by(w, br.dw, br.inv_dw), f, br.planes, br.plane0)) {
             uint32_t keys[4];
             const int nk = row_keys(g, f, keys);
             for (int q = 0; q < nk; ++q) {
@@ -567,9 +578,11 @@ k_scatter(GridGeom g, uint32_t n_all, uint32_t F, uint32_t P, const double *__re
             wave_row_sums<VMODE>(i - (threadIdx.x & 63u), n, P, vr, vi, wt, w_re, w_im);
         Foot f;
         if (i >= n) continue;
-        pu[k] = uu[i] / g.dx;
-        pv[k] = vv[i] / g.dx;
-        if (!footprint(g, pu[k], pv[k], ww[i] / br.dw, f, br.planes, br.plane0)) continue;
+        pu[k] = div_by(uu[i], g.dx, g.inv_dx);
+        pv[k] = div_by(vv[i], g.dx, g.inv_dx);
+        if (!footprint(g, pu[k], pv[k], div_by(ww[i], br.dw, br.inv_dw), f, br.planes,
+                       br.plane0))
+            continue;
         // Channel/polarization sum, w-stacking.cu:512-527 (weights indexed
         // row*P + p, shared across channels; float product first).
         const uint32_t FP = F * P;

@@ -68,7 +68,19 @@ struct GridGeom {
     double cmax;            // largest per-cell convolution weight (a bound)
     int zrun;               // empty buckets cleared per zero task (1..4)
     int smax;               // unrolled footprint length of the tile kernel (>= support)
+    // 1/dx and 1/dw when dx / dw are powers of two (then x * inv is exactly
+    // the IEEE quotient x / d), else 0 (divide).
+    double inv_dx, inv_dw;
 };
+
+// 1/d if d is a power of two whose reciprocal is exact, else 0.
+inline double exact_inverse(double d)
+{
+    int e = 0;
+    if (!(d > 0.0) || __builtin_frexp(d, &e) != 0.5) return 0.0;
+    const double inv = 1.0 / d;
+    return (__builtin_isfinite(inv) && 1.0 / inv == d) ? inv : 0.0;
+}
 
 // Workspace of the binned gridder, carved out of one device allocation.
 struct GridWork {
@@ -104,7 +116,7 @@ struct BatchJob {
     uint32_t n;             // rows
     uint32_t row0;          // first virtual row
     int plane0, planes;     // the job's planes in the group's plane stack
-    double dw;
+    double dw, inv_dw;      // inv_dw: exact_inverse(dw)
 };
 // Rows per block of the count kernel: batch jobs start at multiples of it.
 uint32_t batch_row_align();
