@@ -131,6 +131,14 @@ __device__ __forceinline__ void uv_range(const GridGeom &g, double pu, double pv
     f.kmax = (f.gv < g.ny - g.kl) ? f.gv + g.kl : g.ny - 1;
 }
 
+// A pointer known to address global memory: loads through it are global
+// loads, not flat ones (pointers read from a job table are otherwise flat).
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *as_global(const T *p)
+{
+    return (const __attribute__((address_space(1))) T *)p;
+}
+
 // x / d, the reference's IEEE division (w-stacking.cu:464-466).  When d is a
 // power of two with an exact reciprocal inv (inv != 0), x * inv is that same
 // correctly rounded quotient (both are the exact value x * 2^-k, rounded
@@ -240,9 +248,9 @@ k_count(GridGeom g, uint32_t n_all, const double *__restrict__ uu_all,
     const BlockRows br = block_rows(g, blockIdx.x * (uint32_t)kBinRows, n_all, uu_all, vv_all,
                                     ww_all, nullptr, nullptr, nullptr, BATCH ? jobs : nullptr,
                                     njobs);
-    const double *__restrict__ uu = br.uu;
-    const double *__restrict__ vv = br.vv;
-    const double *__restrict__ ww = br.ww;
+    const auto *__restrict__ uu = as_global(br.uu);
+    const auto *__restrict__ vv = as_global(br.vv);
+    const auto *__restrict__ ww = as_global(br.ww);
     const uint32_t n = br.n, base = br.base;
     // The next row's coordinates are loaded before the current row is hashed,
     // so their latency overlaps the LDS work.
@@ -557,14 +565,45 @@ k_scatter(GridGeom g, uint32_t n_all, uint32_t F, uint32_t P, const double *__re
     const uint32_t n = br.n;
     __shared__ uint32_t hkey[kHashSlots], hcnt[kHashSlots], hbase[kHashSlots];
     __shared__ unsigned long long smax[kScatterThreads / 64];
+    const uint32_t base = br.base;
+    // The read path is chosen on the host, so each instantiation carries one.
+    constexpr bool vec4 = VMODE == 1;
+    const uint32_t FP = F * P;
+    // Every coordinate of the thread's rows (and, with one value per row,
+    // its visibility and weight) is requested before any is used, so the
+    // loads of both rows overlap instead of each waiting on the arithmetic
+    // of the one before.
+    double ux[kRowsPerThread], vx[kRowsPerThread], wx[kRowsPerThread];
+    float r1[kRowsPerThread], i1[kRowsPerThread], wt1[kRowsPerThread];
+#pragma unroll
+    for (int k = 0; k < kRowsPerThread; ++k) {
+        const uint32_t i = base + k * kScatterThreads + threadIdx.x;
+        ux[k] = vx[k] = wx[k] = 0.0;
+        r1[k] = i1[k] = wt1[k] = 0.f;
+        if (i < n) {
+            ux[k] = as_global(uu)[i];
+            vx[k] = as_global(vv)[i];
+            wx[k] = as_global(ww)[i];
+            if (VMODE == 0 && FP == 1u) {
+                r1[k] = as_global(vr)[i];
+                i1[k] = as_global(vi)[i];
+                wt1[k] = as_global(wt)[i];
+            }
+        }
+    }
+    double w_re[kRowsPerThread], w_im[kRowsPerThread];
+#pragma unroll
+    for (int k = 0; k < kRowsPerThread; ++k) {
+        w_re[k] = w_im[k] = 0.0;
+        if constexpr (VMODE >= 2)  // whole wave, before any lane leaves
+            wave_row_sums<VMODE>(base + k * kScatterThreads + threadIdx.x - (threadIdx.x & 63u),
+                                 n, P, vr, vi, wt, w_re[k], w_im[k]);
+    }
     for (int i = threadIdx.x; i < kHashSlots; i += kScatterThreads) {
         hkey[i] = kEmpty;
         hcnt[i] = 0;
     }
     __syncthreads();
-    const uint32_t base = br.base;
-    // The read path is chosen on the host, so each instantiation carries one.
-    constexpr bool vec4 = VMODE == 1;
     double pu[kRowsPerThread], pv[kRowsPerThread], sre[kRowsPerThread], sim[kRowsPerThread];
     uint32_t item[kRowsPerThread][4];
     int nk[kRowsPerThread];
@@ -573,25 +612,21 @@ k_scatter(GridGeom g, uint32_t n_all, uint32_t F, uint32_t P, const double *__re
         const uint32_t i = base + k * kScatterThreads + threadIdx.x;
         nk[k] = 0;
         pu[k] = pv[k] = sre[k] = sim[k] = 0.0;
-        double w_re = 0.0, w_im = 0.0;
-        if constexpr (VMODE >= 2)  // whole wave, before any lane leaves
-            wave_row_sums<VMODE>(i - (threadIdx.x & 63u), n, P, vr, vi, wt, w_re, w_im);
         Foot f;
         if (i >= n) continue;
-        pu[k] = div_by(uu[i], g.dx, g.inv_dx);
-        pv[k] = div_by(vv[i], g.dx, g.inv_dx);
-        if (!footprint(g, pu[k], pv[k], div_by(ww[i], br.dw, br.inv_dw), f, br.planes,
+        pu[k] = div_by(ux[k], g.dx, g.inv_dx);
+        pv[k] = div_by(vx[k], g.dx, g.inv_dx);
+        if (!footprint(g, pu[k], pv[k], div_by(wx[k], br.dw, br.inv_dw), f, br.planes,
                        br.plane0))
             continue;
         // Channel/polarization sum, w-stacking.cu:512-527 (weights indexed
         // row*P + p, shared across channels; float product first).
-        const uint32_t FP = F * P;
         const size_t vb = (size_t)i * FP;
         const float *wrow = wt + (size_t)i * P;
         double a_re = 0.0, a_im = 0.0;
         if constexpr (VMODE >= 2) {
-            a_re = w_re;
-            a_im = w_im;
+            a_re = w_re[k];
+            a_im = w_im[k];
         } else if (vec4) {
             // Same sum in the same element order, read 16 bytes at a time.
             const float4 *r4 = reinterpret_cast<const float4 *>(vr + vb);
@@ -609,6 +644,13 @@ k_scatter(GridGeom g, uint32_t n_all, uint32_t F, uint32_t P, const double *__re
                         a_im += (double)wi;
                     }
                 }
+            }
+        } else if (FP == 1u) {
+            if (!isnan(r1[k])) {
+                const float wr = wt1[k] * r1[k];
+                const float wi = wt1[k] * i1[k];
+                a_re += (double)wr;
+                a_im += (double)wi;
             }
         } else {
             for (uint32_t fr = 0; fr < F; ++fr) {
@@ -662,11 +704,20 @@ k_scatter(GridGeom g, uint32_t n_all, uint32_t F, uint32_t P, const double *__re
             if (m) atomicMax(reinterpret_cast<unsigned long long *>(meta + 6), m);
         }
     }
-    // One returning global atomic per distinct key, all issued together.
-    for (int i = threadIdx.x; i < kHashSlots; i += kScatterThreads) {
+    // One returning global atomic per distinct key.  A thread's atomics are
+    // all issued before any result is stored, so their round trips overlap
+    // (in a rolled loop each waits for the one before).
+    constexpr int kSlotsPerThread = kHashSlots / kScatterThreads;
+    static_assert(kHashSlots % kScatterThreads == 0, "hash slots per thread");
+    uint32_t got[kSlotsPerThread];
+#pragma unroll
+    for (int t = 0; t < kSlotsPerThread; ++t) {
+        const int i = threadIdx.x + t * kScatterThreads;
         const uint32_t c = hcnt[i];
-        if (c) hbase[i] = atomicAdd(&cursor[hkey[i]], c);
+        got[t] = c ? atomicAdd(&cursor[hkey[i]], c) : 0u;
     }
+#pragma unroll
+    for (int t = 0; t < kSlotsPerThread; ++t) hbase[threadIdx.x + t * kScatterThreads] = got[t];
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kRowsPerThread; ++k) {
