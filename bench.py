@@ -106,12 +106,21 @@ def timed(fn, steps, torch):
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
-def run_path(ds, ins, precision, steps, warmup, torch, tile=(0, 0), rpt=0, unroll=0):
+def run_path(ds, ins, precision, steps, warmup, torch, tile=(0, 0), rpt=0, unroll=0,
+             image_shape=None):
+    """Gridding step and image chain of one precision.  The image chain runs on
+    its own plan when image_shape (tile, rows per task, unroll: the shape tuned
+    for imaging) differs from the gridding shape; both plans share the grid."""
     from rick_amd import imaging
     cfg = ds.cfg
     F, P = cfg.n_chan, cfg.n_pol
     plan = imaging.plan_for(ds, precision=precision, tile=tile, rows_per_task=rpt,
                             unroll=unroll)
+    iplan = plan
+    if image_shape and tuple(image_shape) != (tuple(tile), rpt, unroll):
+        itile, irpt, iunroll = image_shape
+        iplan = imaging.plan_for(ds, precision=precision, tile=itile, rows_per_task=irpt,
+                                 unroll=iunroll)
     grid = plan.new_grid()
     npix = cfg.grid * cfg.grid
     re = torch.empty(npix, dtype=torch.float64, device="cuda:0")
@@ -121,7 +130,7 @@ def run_path(ds, ins, precision, steps, warmup, torch, tile=(0, 0), rpt=0, unrol
         plan.grid(*ins, F, P, out=grid)
 
     def image_step():
-        plan.image(*ins, F, P, grid=grid, image_real=re, image_imag=im)
+        iplan.image(*ins, F, P, grid=grid, image_real=re, image_imag=im)
 
     for _ in range(warmup):
         grid_step()
@@ -131,7 +140,7 @@ def run_path(ds, ins, precision, steps, warmup, torch, tile=(0, 0), rpt=0, unrol
 
     # Per-kernel durations (HIP events on the launch stream), for the
     # gridding step (dense grid) and for the image chain (sparse grid).
-    def stage_profile(fn):
+    def stage_profile(fn, plan):
         plan.profile(True)
         for _ in range(steps):
             fn()
@@ -141,11 +150,11 @@ def run_path(ds, ins, precision, steps, warmup, torch, tile=(0, 0), rpt=0, unrol
                     "ms_per_step": ms / steps}
                 for k, (ms, n) in st.items() if n}
 
-    stages = stage_profile(grid_step)
+    stages = stage_profile(grid_step, plan)
     stats = plan.grid_stats()
-    image_stages = stage_profile(image_step)
+    image_stages = stage_profile(image_step, iplan)
     img = (re.cpu().numpy() + 1j * im.cpu().numpy())
-    return plan, grid_ms, image_ms, stages, image_stages, stats, img
+    return plan, iplan, grid_ms, image_ms, stages, image_stages, stats, img
 
 
 def host_cores():
@@ -504,20 +513,28 @@ def main():
             tuned = autotune.lookup(cfg.grid, cfg.planes, cfg.support, prec, cfg.n_rows)
             if tuned:
                 tile, rpt, unroll = tuned
-        log(f"{name}: tile {tile} rows/task {rpt} unroll {unroll or 'default'}; warmup "
-            f"{args.warmup}, steps {args.steps}")
-        plan, gms, ims, stages, istages, stats, img = run_path(ds, ins, prec, args.steps,
-                                                               args.warmup, torch, tile, rpt,
-                                                               unroll)
+        ishape = None
+        if not (args.tile or args.tune):
+            ituned = autotune.lookup(cfg.grid, cfg.planes, cfg.support, prec, cfg.n_rows,
+                                     kind="image")
+            if ituned:
+                ishape = (ituned[0], ituned[1], ituned[2] or unroll)
+        log(f"{name}: tile {tile} rows/task {rpt} unroll {unroll or 'default'}"
+            + (f"; image chain tile {ishape[0]} rows/task {ishape[1]}" if ishape else "")
+            + f"; warmup {args.warmup}, steps {args.steps}")
+        plan, iplan, gms, ims, stages, istages, stats, img = run_path(
+            ds, ins, prec, args.steps, args.warmup, torch, tile, rpt, unroll, ishape)
         res[name] = {"grid_ms": gms, "image_ms": ims, "grid_vis_per_s": vis / (gms * 1e-3),
                      "stages": stages, "image_stages": istages, "grid_stats": stats,
                      "img": img,
                      "grid_bytes": plan.grid_bytes,
                      "tile": [plan.desc.tile_u or 64, plan.desc.tile_v or 64],
                      "rows_per_task": plan.desc.rows_per_task or 4096,
-                     "footprint_unroll": plan.desc.footprint_unroll or None}
+                     "footprint_unroll": plan.desc.footprint_unroll or None,
+                     "image_tile": [iplan.desc.tile_u or 64, iplan.desc.tile_v or 64],
+                     "image_rows_per_task": iplan.desc.rows_per_task or 4096}
         log(f"{name}: grid {gms:.3f} ms ({vis / gms / 1e6:.3f} Gvis/s), image {ims:.3f} ms")
-        del plan
+        del plan, iplan
         torch.cuda.empty_cache()
 
     copy_gbs = copy_bandwidth(torch)
@@ -576,6 +593,8 @@ def main():
         out[name] = {"grid_ms": v["grid_ms"], "image_ms": v["image_ms"],
                      "grid_vis_per_s": v["grid_vis_per_s"], "grid_stats": v["grid_stats"],
                      "tile": v["tile"], "rows_per_task": v["rows_per_task"],
+                     "image_tile": v["image_tile"],
+                     "image_rows_per_task": v["image_rows_per_task"],
                      "stages_ms_per_step": {k: round(s["ms_per_step"], 4)
                                             for k, s in v["stages"].items()},
                      "image_stages_ms_per_step": {k: round(s["ms_per_step"], 4)
