@@ -252,38 +252,38 @@ k_count(GridGeom g, uint32_t n_all, const double *__restrict__ uu_all,
     const auto *__restrict__ vv = as_global(br.vv);
     const auto *__restrict__ ww = as_global(br.ww);
     const uint32_t n = br.n, base = br.base;
-    // The next row's coordinates are loaded before the current row is hashed,
-    // so their latency overlaps the LDS work.
-    uint32_t i = base + threadIdx.x;
-    double u = 0.0, v = 0.0, w = 0.0;
-    if (i < n) {
-        u = uu[i];
-        v = vv[i];
-        w = ww[i];
-    }
-    for (int k = 0; k < kBinRowsPerThread; ++k) {
-        const uint32_t in = i + kBinThreads;
-        double un = 0.0, vn = 0.0, wn = 0.0;
-        if (k + 1 < kBinRowsPerThread && in < n) {
-            un = uu[in];
-            vn = vv[in];
-            wn = ww[in];
-        }
-        Foot f;
-        if (i < n && footprint(g, div_by(u, g.dx, g.inv_dx), div_by(v, g.dx, g.inv_dx),
-                               div_by(w, br.dw, br.inv_dw), f, br.planes, br.plane0)) {
-            uint32_t keys[4];
-            const int nk = row_keys(g, f, keys);
-            for (int q = 0; q < nk; ++q) {
-                const uint32_t h = hinsert_peek<kBinHashBits>(hkey, keys[q]);
-                if (h == kEmpty) atomicAdd(&counts[keys[q]], 1u);
-                else atomicAdd(&hcnt[h], 1u);
+    // A thread's rows are taken kCountBatch at a time, all their coordinates
+    // requested before any is hashed: one memory round trip per batch rather
+    // than one per row.
+    constexpr int kCountBatch = 4;
+    static_assert(kBinRowsPerThread % kCountBatch == 0, "row batches per thread");
+    for (int k0 = 0; k0 < kBinRowsPerThread; k0 += kCountBatch) {
+        double u[kCountBatch], v[kCountBatch], w[kCountBatch];
+#pragma unroll
+        for (int j = 0; j < kCountBatch; ++j) {
+            const uint32_t i = base + (uint32_t)(k0 + j) * kBinThreads + threadIdx.x;
+            u[j] = v[j] = w[j] = 0.0;
+            if (i < n) {
+                u[j] = uu[i];
+                v[j] = vv[i];
+                w[j] = ww[i];
             }
         }
-        i = in;
-        u = un;
-        v = vn;
-        w = wn;
+#pragma unroll
+        for (int j = 0; j < kCountBatch; ++j) {
+            const uint32_t i = base + (uint32_t)(k0 + j) * kBinThreads + threadIdx.x;
+            Foot f;
+            if (i < n && footprint(g, div_by(u[j], g.dx, g.inv_dx), div_by(v[j], g.dx, g.inv_dx),
+                                   div_by(w[j], br.dw, br.inv_dw), f, br.planes, br.plane0)) {
+                uint32_t keys[4];
+                const int nk = row_keys(g, f, keys);
+                for (int q = 0; q < nk; ++q) {
+                    const uint32_t h = hinsert_peek<kBinHashBits>(hkey, keys[q]);
+                    if (h == kEmpty) atomicAdd(&counts[keys[q]], 1u);
+                    else atomicAdd(&hcnt[h], 1u);
+                }
+            }
+        }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kBinSlots; i += kBinThreads) {
