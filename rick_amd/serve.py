@@ -248,12 +248,21 @@ class Server:
                 still.append((req, handle, need))
         self.running = still
 
-    def run(self, requests, keep_results=False, poll=0.0002):
-        """Replay `requests` (sorted by arrival) in real time; returns them."""
+    def run(self, requests, keep_results=False, poll=0.0002, progress=0.0, log=None):
+        """Replay `requests` (sorted by arrival) in real time; returns them.
+        With progress > 0, `log` (default: stderr) gets a status line every
+        `progress` seconds: elapsed time, completed, queued and running jobs."""
         self.t0 = self.clock()
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        next_report = progress
         while pending or self.queue or self.running:
             now = self.clock() - self.t0
+            if progress > 0 and now >= next_report:
+                msg = (f"[serve] t={now:.0f} s: {len(self.finished)} finished, "
+                       f"{len(self.queue)} queued, {len(self.running)} running, "
+                       f"{len(pending)} to arrive")
+                (log or (lambda m: print(m, file=sys.stderr, flush=True)))(msg)
+                next_report += progress
             while pending and pending[0].arrival <= now:
                 self.submit(pending.popleft(), now)
             self._reap(now, keep_results)
@@ -310,6 +319,10 @@ def main():
     ap.add_argument("--rate", type=float, default=400.0, help="requests per second")
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--budget-gb", type=float, default=256.0)
+    ap.add_argument("--paths", choices=["both", "fp32", "fp64"], default="both",
+                    help="FP32 continuous batching, FP64 sequential, or both")
+    ap.add_argument("--progress", type=float, default=30.0,
+                    help="seconds between status lines on stderr (0: none)")
     args = ap.parse_args()
     cfgs, weights = workload()
     datasets = [synth.make_dataset(c, device="cuda:0") for c in cfgs]
@@ -319,19 +332,19 @@ def main():
     out = {"requests": len(reqs), "rate_per_s": args.rate, "duration_s": args.duration,
            "slots": args.slots, "budget_gb": args.budget_gb}
     # Warm the plan pools (plan creation allocates) outside the measured runs.
-    for prec in (FP32, FP64):
+    precs = {"both": (FP32, FP64), "fp32": (FP32,), "fp64": (FP64,)}[args.paths]
+    for prec in precs:
         ex = GpuExecutor(datasets, prec, slots=1 if prec == FP64 else args.slots)
         Server(ex, budget, 1).run([Request(i, i, 0.0) for i in range(len(cfgs))])
         if prec == FP32:
-            fp32_ex = ex
+            srv = Server(ex, budget, args.slots)
+            out["fp32_continuous"] = latency_stats(srv.run(
+                [Request(r.rid, r.key, r.arrival) for r in reqs], progress=args.progress))
         else:
-            fp64_ex = ex
-    srv = Server(fp32_ex, budget, args.slots)
-    out["fp32_continuous"] = latency_stats(srv.run([Request(r.rid, r.key, r.arrival)
-                                                    for r in reqs]))
-    seq = Server(fp64_ex, budget, 1)
-    out["fp64_sequential"] = latency_stats(seq.run([Request(r.rid, r.key, r.arrival)
-                                                    for r in reqs]))
+            seq = Server(ex, budget, 1)
+            out["fp64_sequential"] = latency_stats(seq.run(
+                [Request(r.rid, r.key, r.arrival) for r in reqs], progress=args.progress))
+        del ex
     print(json.dumps(out))
 
 

@@ -185,3 +185,21 @@ def test_gpu_executor_returns_slot_when_acquire_fails():
         with pytest.raises(MemoryError):
             ex.start(serve.Request(0, 0, 0.0))
     assert sorted(ex.free_streams) == [0, 1]
+
+
+def test_progress_lines_during_a_run():
+    reqs = [serve.Request(i, 0, 0.01 * i) for i in range(50)]
+    clock = FakeClock()
+    ex = FakeExecutor(clock, {0: 0.02}, {0: 1}, ())
+    srv = serve.Server(ex, 10**9, 4, clock=clock)
+    lines = []
+    srv.run(reqs, progress=0.1, log=lines.append)
+    assert all(r.status == "done" for r in reqs)
+    # about one line per 0.1 s of the ~0.5 s run, each with the counts
+    assert 3 <= len(lines) <= 8
+    assert all(line.startswith("[serve] t=") and "finished" in line for line in lines)
+    clock2 = FakeClock()
+    quiet = serve.Server(FakeExecutor(clock2, {0: 0.02}, {0: 1}, ()), 10**9, 4, clock=clock2)
+    silent = []
+    quiet.run([serve.Request(i, 0, 0.01 * i) for i in range(10)], log=silent.append)
+    assert silent == []
