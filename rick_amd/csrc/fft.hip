@@ -192,7 +192,7 @@ struct GlobalSink {
         o.x *= scale;
         o.y *= scale;
         if (g < ln->nvalid)
-            gout[ln->out_base + (size_t)g * ln->out_line + (size_t)n * ln->out_el] = o;
+            store_stream2(o, &gout[ln->out_base + (size_t)g * ln->out_line + (size_t)n * ln->out_el]);
     }
 };
 
@@ -677,8 +677,8 @@ k_fft_cols_phase(ColArgs a, int planes_in, const C2<T> *__restrict__ tw,
                         vim = vim * pg.dwnorm * nf;
                     }
                 }
-                ore[pix] = vre;
-                oim[pix] = vim;
+                store_stream(vre, &ore[pix]);
+                store_stream(vim, &oim[pix]);
             }
         }
     }

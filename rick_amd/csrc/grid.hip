@@ -901,13 +901,13 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, uint32_t *__restrict__ meta,
                     constexpr int HU = TU / 2;
                     for (int idx = threadIdx.x; idx < nke * HU; idx += kTileThreads) {
                         const int kk = idx / HU, jj = idx % HU;
-                        if (2 * jj < nje) t4[(size_t)kk * (size_t)(g.nx / 2) + jj] = z4;
+                        if (2 * jj < nje) store_stream4(z4, &t4[(size_t)kk * (size_t)(g.nx / 2) + jj]);
                     }
                     continue;
                 }
                 for (int idx = threadIdx.x; idx < nke * TU; idx += kTileThreads) {
                     const int kk = idx / TU, jj = idx % TU;
-                    if (jj < nje) t0[(size_t)kk * g.nx + jj] = z;
+                    if (jj < nje) store_stream2(z, &t0[(size_t)kk * g.nx + jj]);
                 }
             }
             continue;
@@ -1099,7 +1099,7 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, uint32_t *__restrict__ meta,
                         v.x = o.x + v.x;
                         v.y = o.y + v.y;
                     }
-                    g2[c] = v;
+                    store_stream2(v, &g2[c]);
                 }
                 if (threadIdx.x == 0)
                     __hip_atomic_store(&split_ticket[slot], 0u, __ATOMIC_RELAXED,
@@ -1135,7 +1135,7 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, uint32_t *__restrict__ meta,
                     v.x = o.x + v.x;
                     v.y = o.y + v.y;
                 }
-                *g2 = v;
+                store_stream2(v, g2);
             }
         }
     }

@@ -21,6 +21,31 @@ void clear_status();
 hipError_t launch_split_complex(const void *grid, int precision, size_t n, double *re, double *im,
                                 hipStream_t s);
 
+// Streaming (non-temporal) stores for the large write streams: the grid
+// and the transformed planes are many times the last-level cache and are not
+// read again before being evicted, so keeping them out of the caches leaves
+// those for data that is re-read (the next call's coordinates, twiddles,
+// records).  The builtin takes native vector types, hence the conversions.
+template <typename S>
+__device__ __forceinline__ void store_stream(S v, S *p)
+{
+    __builtin_nontemporal_store(v, p);
+}
+template <typename V2>
+__device__ __forceinline__ void store_stream2(V2 v, V2 *p)
+{
+    using S = decltype(v.x);
+    typedef S nv __attribute__((ext_vector_type(2)));
+    const nv x = {v.x, v.y};
+    __builtin_nontemporal_store(x, reinterpret_cast<nv *>(p));
+}
+__device__ __forceinline__ void store_stream4(float4 v, float4 *p)
+{
+    typedef float nv __attribute__((ext_vector_type(4)));
+    const nv x = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(x, reinterpret_cast<nv *>(p));
+}
+
 // Table oversampling of the reference (w-stacking.cu:253).
 constexpr int kOversample = 5;
 // Truncated pi literal of the reference (w-stacking.h:6).
