@@ -215,8 +215,8 @@ __device__ __forceinline__ void load_first(C2<T> *vals, int tid, const Lines &ln
                 g < ln.nvalid && (ln.occ == nullptr || ln.occ[g * ln.occ_stride + n / ln.occ_tu]) &&
                 (ln.rowlive == nullptr ||
                  ln.rowlive[(ln.rl_row0 + n * ln.rl_rstride) >> ln.rl_shift]);
-            vals[i * R + r] = live ? gin[ln.in_base + (size_t)g * ln.in_line + (size_t)n * ln.in_el]
-                                   : C2<T>{T(0), T(0)};
+            const size_t at = ln.in_base + (size_t)g * ln.in_line + (size_t)n * ln.in_el;
+            vals[i * R + r] = live ? load_stream2(&gin[at]) : C2<T>{T(0), T(0)};
         }
     }
 }
@@ -263,8 +263,8 @@ __device__ __forceinline__ void pass(C2<T> *vals, int tid, const Lines &ln, int 
                             (ln.occ == nullptr || ln.occ[g * ln.occ_stride + n / ln.occ_tu]) &&
                             (ln.rowlive == nullptr ||
                              ln.rowlive[(ln.rl_row0 + n * ln.rl_rstride) >> ln.rl_shift]);
-                        v[r] = live ? gin[ln.in_base + (size_t)g * ln.in_line +
-                                          (size_t)n * ln.in_el]
+                        v[r] = live ? load_stream2(&gin[ln.in_base + (size_t)g * ln.in_line +
+                                                        (size_t)n * ln.in_el])
                                     : C2<T>{T(0), T(0)};
                     }
                 }

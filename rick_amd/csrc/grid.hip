@@ -506,8 +506,8 @@ __device__ __forceinline__ void wave_row_sums(uint32_t r0, uint32_t n, uint32_t 
         float wv[4] = {0.f, 0.f, 0.f, 0.f};
         if (row < n) {
             const size_t e = (size_t)row * L + (size_t)q;
-            ra = r4[e];
-            ia = i4[e];
+            ra = load_stream4(&r4[e]);
+            ia = load_stream4(&i4[e]);
             const float *wrow = wt + (size_t)row * P;
 #pragma unroll
             for (int c = 0; c < 4; ++c) wv[c] = wrow[P == 1 ? 0u : (uint32_t)(4 * q + c) % P];
@@ -632,7 +632,7 @@ k_scatter(GridGeom g, uint32_t n_all, uint32_t F, uint32_t P, const double *__re
             const float4 *r4 = reinterpret_cast<const float4 *>(vr + vb);
             const float4 *i4 = reinterpret_cast<const float4 *>(vi + vb);
             for (uint32_t e4 = 0; e4 < FP / 4; ++e4) {
-                const float4 ra = r4[e4], ia = i4[e4];
+                const float4 ra = load_stream4(&r4[e4]), ia = load_stream4(&i4[e4]);
                 const float rr[4] = {ra.x, ra.y, ra.z, ra.w}, ii[4] = {ia.x, ia.y, ia.z, ia.w};
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {

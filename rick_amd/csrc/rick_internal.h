@@ -45,6 +45,25 @@ __device__ __forceinline__ void store_stream4(float4 v, float4 *p)
     const nv x = {v.x, v.y, v.z, v.w};
     __builtin_nontemporal_store(x, reinterpret_cast<nv *>(p));
 }
+// Streaming loads for data read once (the planes a transform pass reads, the
+// visibilities the scatter sums).
+template <typename V2>
+__device__ __forceinline__ V2 load_stream2(const V2 *p)
+{
+    using S = decltype(p->x);
+    typedef S nv __attribute__((ext_vector_type(2)));
+    const nv x = __builtin_nontemporal_load(reinterpret_cast<const nv *>(p));
+    V2 r;
+    r.x = x.x;
+    r.y = x.y;
+    return r;
+}
+__device__ __forceinline__ float4 load_stream4(const float4 *p)
+{
+    typedef float nv __attribute__((ext_vector_type(4)));
+    const nv x = __builtin_nontemporal_load(reinterpret_cast<const nv *>(p));
+    return make_float4(x.x, x.y, x.z, x.w);
+}
 
 // Table oversampling of the reference (w-stacking.cu:253).
 constexpr int kOversample = 5;
