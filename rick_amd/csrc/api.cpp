@@ -462,7 +462,7 @@ int rick_plan_image(rick_plan p, unsigned int n, unsigned int F, unsigned int P,
                          vi, wt, grid, 0, s, &p->timing, !sparse));
     RICK_HIP(launch_image_chain(p->fft, p->d.precision, grid, norm, s, &p->timing,
                                 sparse ? &p->geom : nullptr, sparse ? p->work.counts : nullptr,
-                                sparse ? p->work.band : nullptr, p->phase, re, im));
+                                sparse ? p->work.rowlive : nullptr, p->phase, re, im));
     return RICK_OK;
 }
 
@@ -619,7 +619,7 @@ int rick_batch_run(rick_batch b, void *stream)
     const double norm = 1.0 / (double)((double)p->geom.nx * (double)p->geom.ny);
     RICK_HIP(launch_image_chain(p->fft, p->d.precision, b->grid, norm, s, &p->timing,
                                 sparse ? &p->geom : nullptr, sparse ? p->work.counts : nullptr,
-                                sparse ? p->work.band : nullptr, p->phase, nullptr, nullptr,
+                                sparse ? p->work.rowlive : nullptr, p->phase, nullptr, nullptr,
                                 b->pjobs_d, b->njobs));
     return RICK_OK;
 }

@@ -160,11 +160,10 @@ struct Lines {
     // is taken as zero (cells of empty gridder tiles are never written).
     const unsigned char *occ;
     int occ_stride, occ_tu;
-    // Optional row-band occupancy of the input (columns): element n is read
-    // only if rowlive[(rl_row0 + n * rl_rstride) >> rl_shift] is non-zero, else
-    // it is zero (rows of empty bands are skipped by the row transform).
+    // Optional row liveness of the input (columns): element n is read only if
+    // rowlive[n] is non-zero, else it is zero (grid rows without a non-zero
+    // cell are skipped by the row transform).
     const unsigned char *rowlive;
-    int rl_row0, rl_rstride, rl_shift;
 };
 
 // Butterfly b of a pass with L/R butterflies per line: rows map consecutive
@@ -213,8 +212,7 @@ __device__ __forceinline__ void load_first(C2<T> *vals, int tid, const Lines &ln
             const int n = j + r * LR;
             const bool live =
                 g < ln.nvalid && (ln.occ == nullptr || ln.occ[g * ln.occ_stride + n / ln.occ_tu]) &&
-                (ln.rowlive == nullptr ||
-                 ln.rowlive[(ln.rl_row0 + n * ln.rl_rstride) >> ln.rl_shift]);
+                (ln.rowlive == nullptr || ln.rowlive[n]);
             const size_t at = ln.in_base + (size_t)g * ln.in_line + (size_t)n * ln.in_el;
             vals[i * R + r] = live ? load_stream2(&gin[at]) : C2<T>{T(0), T(0)};
         }
@@ -261,8 +259,7 @@ __device__ __forceinline__ void pass(C2<T> *vals, int tid, const Lines &ln, int 
                         const bool live =
                             g < ln.nvalid &&
                             (ln.occ == nullptr || ln.occ[g * ln.occ_stride + n / ln.occ_tu]) &&
-                            (ln.rowlive == nullptr ||
-                             ln.rowlive[(ln.rl_row0 + n * ln.rl_rstride) >> ln.rl_shift]);
+                            (ln.rowlive == nullptr || ln.rowlive[n]);
                         v[r] = live ? load_stream2(&gin[ln.in_base + (size_t)g * ln.in_line +
                                                         (size_t)n * ln.in_el])
                                     : C2<T>{T(0), T(0)};
@@ -345,47 +342,34 @@ constexpr int lds_elems(int L, bool rows)
 // Occupancy flags per workgroup: G rows x (L/32) tiles of >= 32 cells.
 constexpr int oc_bytes(int L) { return lines_per_wg(L) * (L / 32 > 0 ? L / 32 : 1); }
 
-// Tile occupancy of a sparse grid (gridder buckets): counts of the bucket
-// (plane, tile row, tile column); zero means the tile was never written.
-// band[plane * tiles_v + t] is non-zero iff tile row t of the plane holds an
-// occupied tile: the grid rows of an empty band are zero, so the row
-// transform skips them (writes nothing) and later column passes read them as
-// zero.
+// Occupancy of a sparse grid: counts of the gridder bucket (plane, tile row,
+// tile column), zero where the tile was never written, and rows[plane * ny +
+// v], non-zero iff grid row v of the plane holds a non-zero cell.  The row
+// transform skips the other rows (writes nothing) and later column passes
+// read them as zero.
 struct Occupancy {
     const uint32_t *counts;
-    uint32_t *band;
+    const unsigned char *rows;
     int tu, tv, tiles_u, tiles_v, ny;
 };
 
-__global__ void __launch_bounds__(256) k_band_occ(Occupancy oc, int planes)
-{
-    const int bi = blockIdx.x * 256 + threadIdx.x;
-    if (bi >= planes * oc.tiles_v) return;
-    const uint32_t *c = oc.counts + (size_t)bi * oc.tiles_u;
-    uint32_t any = 0;
-    for (int t = 0; t < oc.tiles_u; ++t) any |= c[t];
-    oc.band[bi] = any != 0;
-}
-
-// Row-band flags of one plane for the column kernels (null: dense input).
+// Row flags of the grid for the column kernels (null: dense input).
 struct ColOcc {
-    const uint32_t *band;
-    int tiles_v, tv_shift;
+    const unsigned char *rows;
+    int ny;
 };
 
-// Loads the plane's band flags into LDS and points the input at them.
+// Loads the flags of the L input rows of this workgroup (row0 + n * rstride
+// of the plane) into LDS and points the input at them.
 __device__ __forceinline__ void col_rowlive(const ColOcc &co, int plane, int row0, int rstride,
-                                            unsigned char *rl, Lines &ln)
+                                            int L, unsigned char *rl, Lines &ln)
 {
     ln.rowlive = nullptr;
-    if (co.band == nullptr) return;
-    for (int t = threadIdx.x; t < co.tiles_v; t += blockDim.x)
-        rl[t] = co.band[(size_t)plane * co.tiles_v + t] != 0;
+    if (co.rows == nullptr) return;
+    const unsigned char *pr = co.rows + (size_t)plane * co.ny + row0;
+    for (int n = threadIdx.x; n < L; n += blockDim.x) rl[n] = pr[(size_t)n * rstride];
     __syncthreads();
     ln.rowlive = rl;
-    ln.rl_row0 = row0;
-    ln.rl_rstride = rstride;
-    ln.rl_shift = co.tv_shift;
 }
 
 // Rows: workgroup w handles rows w*G .. w*G+G-1 of the planes*ny rows.
@@ -413,15 +397,11 @@ k_fft_rows(int rows_total, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T>
     ln.tw_set = 0;
     ln.occ = nullptr;
     ln.rowlive = nullptr;
-    if (oc.band) {
-        // Uniform across the workgroup: all its rows in empty bands -> the
-        // output rows are zero and are neither computed nor written.
+    if (oc.rows) {
+        // Uniform across the workgroup: none of its rows holds a non-zero
+        // cell -> the output rows are zero and are neither computed nor written.
         int live = 0;
-        for (int g = threadIdx.x; g < ln.nvalid; g += NT) {
-            const size_t row = row0 + g;
-            const int plane = (int)(row / oc.ny), v = (int)(row % oc.ny);
-            live |= oc.band[(size_t)plane * oc.tiles_v + v / oc.tv] != 0;
-        }
+        for (int g = threadIdx.x; g < ln.nvalid; g += NT) live |= oc.rows[row0 + g];
         if (!__syncthreads_or(live)) return;
     }
     if (oc.counts) {
@@ -429,7 +409,7 @@ k_fft_rows(int rows_total, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T>
             const int g = i / oc.tiles_u, t = i % oc.tiles_u;
             const size_t row = row0 + g;
             unsigned char live = 0;
-            if (g < ln.nvalid) {
+            if (g < ln.nvalid && (oc.rows == nullptr || oc.rows[row])) {
                 const int plane = (int)(row / oc.ny), v = (int)(row % oc.ny);
                 live = oc.counts[((size_t)plane * oc.tiles_v + v / oc.tv) * oc.tiles_u + t] != 0;
             }
@@ -462,7 +442,7 @@ k_fft_cols(ColArgs a, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gou
 {
     constexpr bool kSplit = sizeof(T) == 8;   // as in k_fft_rows
     __shared__ C2<T> lds[kSplit ? G * L / 2 : G * L];
-    __shared__ unsigned char rl[256];
+    __shared__ unsigned char rl[L];
     const int c0 = blockIdx.x * G;
     const int s = blockIdx.y;
     const size_t plane = (size_t)blockIdx.z * (size_t)a.nx * (size_t)a.ny;
@@ -475,7 +455,7 @@ k_fft_cols(ColArgs a, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gou
     ln.nvalid = min(G, a.nx - c0);
     ln.tw_set = a.tw_by_set ? s : 0;
     ln.occ = nullptr;
-    col_rowlive(co, blockIdx.z, s * a.in_set_mul, a.in_stride, rl, ln);
+    col_rowlive(co, blockIdx.z, s * a.in_set_mul, a.in_stride, L, rl, ln);
     C2<T> vals[E];
     GlobalSink<T> sink{gout, &ln, tw, scale};
     pass<T, L, G, false, 0, GlobalSink<T>, false, kSplit, E>(vals, threadIdx.x, ln, a.ts, tw,
@@ -551,10 +531,10 @@ k_fft_cols_phase(ColArgs a, int planes_in, const C2<T> *__restrict__ tw,
     constexpr int G = lines_per_wg(L);
     constexpr int NT = G * L / E;
     __shared__ C2<T> lds[lds_elems(L, false)];
-    // Band flags of two planes: with the FP64 prefetch, plane iw + 1's flags
+    // Row flags of two planes: with the FP64 prefetch, plane iw + 1's flags
     // are written while other waves may still be choosing which of plane
     // iw's rows to load, so consecutive planes use different halves.
-    __shared__ unsigned char rl[2][256];
+    __shared__ unsigned char rl[2][L];
     __shared__ double nfl[sizeof(T) == 4 ? G * L : 1];
     const int c0 = blockIdx.x * G;
     const int s = blockIdx.y;
@@ -625,7 +605,7 @@ k_fft_cols_phase(ColArgs a, int planes_in, const C2<T> *__restrict__ tw,
     }
     auto plane_lines = [&](int iw) {
         const int pw = plane0 + iw;   // plane in the grid's (group's) stack
-        if (co.band) col_rowlive(co, pw, s * a.in_set_mul, a.in_stride, rl[iw & 1], ln);
+        if (co.rows) col_rowlive(co, pw, s * a.in_set_mul, a.in_stride, L, rl[iw & 1], ln);
         ln.in_base = (size_t)pw * (size_t)a.nx * (size_t)a.ny +
                      (size_t)s * a.in_set_mul * (size_t)a.nx + c0;
     };
@@ -805,8 +785,9 @@ hipError_t fft_impl(const FftPlan &p, const void *in_v, void *out_v, void *tmp_v
 }
 
 // Image chain: in place, rows first reading only occupied tiles of a sparse
-// grid and skipping rows of empty bands, then (for ny > 512) four-step pass A
-// in place with rows left in transposed order (empty bands read as zero),
+// grid and skipping rows without a non-zero cell, then (for ny > 512)
+// four-step pass A in place with rows left in transposed order (skipped rows
+// read as zero),
 // then the last column pass fused with the phase correction and plane sum,
 // writing the image.
 template <typename T>
@@ -820,12 +801,9 @@ hipError_t image_chain_impl(const FftPlan &p, void *grid_v, double scale_d, hipS
     const CT *twy = static_cast<const CT *>(p.tw_y);
     const int nx = p.nx, ny = p.ny, planes = p.planes;
     ColOcc co{};
-    if (oc.counts && oc.band) {
-        const int nbands = planes * oc.tiles_v;
-        hipLaunchKernelGGL(k_band_occ, dim3((nbands + 255) / 256), dim3(256), 0, s, oc, planes);
-        co.band = oc.band;
-        co.tiles_v = oc.tiles_v;
-        co.tv_shift = __builtin_ctz((unsigned)oc.tv);
+    if (oc.counts && oc.rows) {
+        co.rows = oc.rows;
+        co.ny = ny;
     }
     tm->begin(RICK_STAGE_FFT_ROWS, s);
     hipError_t e = launch_rows<T>(nx, planes * ny, twx, grid, grid, (T)scale_d, s, oc);
@@ -874,7 +852,8 @@ hipError_t launch_fft(const FftPlan &p, int precision, const void *in, void *out
 
 hipError_t launch_image_chain(const FftPlan &p, int precision, void *grid, double scale,
                               hipStream_t s, Timing *tm, const GridGeom *occ_geom,
-                              const uint32_t *occ_counts, uint32_t *occ_band, const PhaseGeom &pg,
+                              const uint32_t *occ_counts, const unsigned char *occ_rows,
+                              const PhaseGeom &pg,
                               double *re, double *im, const PhaseJob *jobs, int njobs)
 {
     Timing off;
@@ -883,7 +862,7 @@ hipError_t launch_image_chain(const FftPlan &p, int precision, void *grid, doubl
     if (occ_geom) {
         if (!occ_counts || p.nx / occ_geom->tu > 256) return hipErrorInvalidValue;
         oc.counts = occ_counts;
-        oc.band = occ_band;
+        oc.rows = occ_rows;
         oc.tu = occ_geom->tu;
         oc.tv = occ_geom->tv;
         oc.tiles_u = occ_geom->tiles_u;

@@ -29,7 +29,10 @@
 //                tasks sit between the accumulation tasks in bucket order, so
 //                the bandwidth-bound clearing shares the launch with the
 //                LDS-bound accumulation.  Every grid cell is written exactly
-//                once, so no memset of the grid is needed.
+//                once, so no memset of the grid is needed.  For the image
+//                chain (sparse overwrite: empty tiles left unwritten) it also
+//                flags each grid row it writes a non-zero cell to, so the FFT
+//                skips the rest.
 //
 // The per-cell weight K[jKer]*K[kKer] is bitwise the reference's; the per-cell
 // value S*c differs from the reference's sum_{f,p} fl(fl32(w*v)*c) only by
@@ -826,7 +829,8 @@ __global__ void __launch_bounds__(kTileThreads)
 k_tile(GridGeom g, const uint4 *__restrict__ tasks, uint32_t *__restrict__ meta,
        const typename RecOf<PREC>::type *__restrict__ rec, const double *__restrict__ table,
        void *grid_v, int accumulate, const uint32_t *__restrict__ offsets,
-       uint32_t *__restrict__ split_ticket, unsigned long long *__restrict__ split_acc)
+       uint32_t *__restrict__ split_ticket, unsigned long long *__restrict__ split_acc,
+       unsigned char *__restrict__ rowlive)
 {
     using T = typename std::conditional<PREC == RICK_FP64, double, float>::type;
     using T2 = typename std::conditional<PREC == RICK_FP64, double2, float2>::type;
@@ -1100,6 +1104,8 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, uint32_t *__restrict__ meta,
                         v.y = o.y + v.y;
                     }
                     store_stream2(v, &g2[c]);
+                    if (rowlive && (v.x != T(0) || v.y != T(0)))
+                        rowlive[(size_t)w * g.ny + (size_t)(k0 + kk)] = 1;
                 }
                 if (threadIdx.x == 0)
                     __hip_atomic_store(&split_ticket[slot], 0u, __ATOMIC_RELAXED,
@@ -1122,6 +1128,8 @@ k_tile(GridGeom g, const uint4 *__restrict__ tasks, uint32_t *__restrict__ meta,
             }
             const size_t c =
                 ((size_t)w * g.ny + (size_t)(k0 + kk)) * (size_t)g.nx + (size_t)(j0 + jj);
+            if (rowlive && (vre != T(0) || vim != T(0)))
+                rowlive[(size_t)w * g.ny + (size_t)(k0 + kk)] = 1;
             if (split) {
                 if (vre != T(0)) unsafeAtomicAdd(&grid[2 * c], vre);
                 if (vim != T(0)) unsafeAtomicAdd(&grid[2 * c + 1], vim);
@@ -1172,7 +1180,7 @@ uint32_t tile_grid(uint32_t task_cap)
 
 template <int PREC, int SMAX>
 hipError_t launch_tile_s(const GridGeom &g, const GridWork &w, const double *table, void *grid,
-                         int accumulate, hipStream_t s, Timing *tm)
+                         int accumulate, unsigned char *rowlive, hipStream_t s, Timing *tm)
 {
     using Rec = typename RecOf<PREC>::type;
     const Rec *rec = static_cast<const Rec *>(w.records);
@@ -1181,7 +1189,7 @@ hipError_t launch_tile_s(const GridGeom &g, const GridWork &w, const double *tab
     hipLaunchKernelGGL((k_tile<PREC, A, B, SMAX, HI>),                                      \
                        dim3(tile_grid<PREC, A, B, SMAX, HI>(w.task_cap)), blk, 0, s,        \
                        g, w.tasks, w.meta, rec, table, grid, accumulate, w.offsets,        \
-                       w.split_ticket, w.split_acc)
+                       w.split_ticket, w.split_acc, rowlive)
 #define RICK_TILE_CASE(A, B)                                                                \
     if (g.tu == A && g.tv == B) {                                                           \
         tm->begin(RICK_STAGE_TILE, s);                                                      \
@@ -1206,13 +1214,13 @@ hipError_t launch_tile_s(const GridGeom &g, const GridWork &w, const double *tab
 // the plan picks the smallest unless told otherwise).
 template <int PREC>
 hipError_t launch_tile(const GridGeom &g, const GridWork &w, const double *table, void *grid,
-                       int accumulate, hipStream_t s, Timing *tm)
+                       int accumulate, unsigned char *rowlive, hipStream_t s, Timing *tm)
 {
     switch (g.smax) {
-        case 7: return launch_tile_s<PREC, 7>(g, w, table, grid, accumulate, s, tm);
-        case 9: return launch_tile_s<PREC, 9>(g, w, table, grid, accumulate, s, tm);
-        case 11: return launch_tile_s<PREC, 11>(g, w, table, grid, accumulate, s, tm);
-        case 15: return launch_tile_s<PREC, 15>(g, w, table, grid, accumulate, s, tm);
+        case 7: return launch_tile_s<PREC, 7>(g, w, table, grid, accumulate, rowlive, s, tm);
+        case 9: return launch_tile_s<PREC, 9>(g, w, table, grid, accumulate, rowlive, s, tm);
+        case 11: return launch_tile_s<PREC, 11>(g, w, table, grid, accumulate, rowlive, s, tm);
+        case 15: return launch_tile_s<PREC, 15>(g, w, table, grid, accumulate, rowlive, s, tm);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1260,6 +1268,7 @@ size_t grid_work_bytes(const GridGeom &g, uint32_t max_points, int precision)
     const size_t rs = precision == RICK_FP64 ? sizeof(RecF64) : sizeof(RecF32);
     size_t b = 0;
     b += align_up(sizeof(uint32_t) * g.nb, 256);           // counts
+    b += align_up((size_t)g.planes * g.ny, 256);           // rowlive (after the counts)
     b += align_up(sizeof(uint32_t) * g.nb, 256);           // cursor
     b += align_up(sizeof(uint32_t) * (g.nb + 1), 256);     // offsets
     b += align_up(sizeof(uint32_t) * kScanQ * nscan, 256); // blocksum
@@ -1267,7 +1276,6 @@ size_t grid_work_bytes(const GridGeom &g, uint32_t max_points, int precision)
     b += align_up(sizeof(uint32_t) * split_cap, 256);      // split_list
     b += align_up(sizeof(uint32_t) * 8, 256);              // meta
     b += align_up(rs * rec_cap, 256);                      // records
-    b += align_up(sizeof(uint32_t) * (size_t)g.planes * g.tiles_v, 256);  // band
     if (precision == RICK_FP32) {
         b += align_up(sizeof(uint32_t) * split_cap, 256);                     // split_ticket
         b += align_up(sizeof(unsigned long long) * 2 * (size_t)g.tu * g.tv * split_cap, 256);
@@ -1288,6 +1296,7 @@ void grid_work_carve(const GridGeom &g, uint32_t max_points, int precision, void
         return q;
     };
     w->counts = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * g.nb));
+    w->rowlive = reinterpret_cast<unsigned char *>(take((size_t)g.planes * g.ny));
     w->cursor = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * g.nb));
     w->offsets = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * (g.nb + 1)));
     w->blocksum = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * kScanQ * nscan));
@@ -1295,7 +1304,6 @@ void grid_work_carve(const GridGeom &g, uint32_t max_points, int precision, void
     w->split_list = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * split_cap));
     w->meta = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * 8));
     w->records = take(rs * rec_cap);
-    w->band = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * (size_t)g.planes * g.tiles_v));
     w->split_ticket = nullptr;
     w->split_acc = nullptr;
     if (precision == RICK_FP32) {
@@ -1320,11 +1328,18 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
     if (tm == nullptr) tm = &off;
     const uint32_t nblk = (n + kRowsPerBlock - 1) / kRowsPerBlock;
     const uint32_t cblk = (n + kBinRows - 1) / kBinRows;
-    const uint32_t fill_blocks = min((g.nb + 255u) / 256u, 4096u);
+    // The counts and the row flags that follow them are cleared together.
+    const uint32_t fill_words =
+        (uint32_t)((reinterpret_cast<const char *>(w.rowlive) - reinterpret_cast<const char *>(w.counts) +
+                    align_up((size_t)g.planes * g.ny, 256)) / sizeof(uint32_t));
+    const uint32_t fill_blocks = min((fill_words + 255u) / 256u, 4096u);
     // Overwrite of a dense grid: runs of empty buckets get zero tasks.
     const int zero = (!accumulate && clear_empty) ? 1 : 0;
+    // Overwrite of a sparse grid: the tile kernel flags the rows it writes
+    // non-zero cells to, for the image chain.
+    unsigned char *rowlive = (!accumulate && !clear_empty) ? w.rowlive : nullptr;
     tm->begin(RICK_STAGE_COUNT_ROWS, s);
-    hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks), dim3(256), 0, s, w.counts, g.nb, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks), dim3(256), 0, s, w.counts, fill_words, 0u);
     if (n > 0) {
         if (jobs)
             hipLaunchKernelGGL(k_count<true>, dim3(cblk), dim3(kBinThreads), 0, s, g, n, uu, vv,
@@ -1384,7 +1399,7 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
                                w.split_list, w.meta, static_cast<double *>(grid));
             tm->end(RICK_STAGE_ZERO_SPLIT, s);
         }
-        return launch_tile<RICK_FP64>(g, w, table_d, grid, accumulate, s, tm);
+        return launch_tile<RICK_FP64>(g, w, table_d, grid, accumulate, rowlive, s, tm);
     }
     // FP32: split buckets accumulate in integer fixed point (order-independent);
     // their grid tiles are cleared for the non-finite fallback.
@@ -1394,7 +1409,7 @@ hipError_t launch_grid(const GridGeom &g, const GridWork &w, int precision,
                            w.split_list, w.meta, static_cast<float *>(grid));
         tm->end(RICK_STAGE_ZERO_SPLIT, s);
     }
-    return launch_tile<RICK_FP32>(g, w, table_d, grid, accumulate, s, tm);
+    return launch_tile<RICK_FP32>(g, w, table_d, grid, accumulate, rowlive, s, tm);
 }
 
 }  // namespace rick

@@ -139,7 +139,10 @@ struct GridWork {
                             //   accumulation tasks (stored before the zero tasks), the
                             //   tile kernel's next task slot (persistent workgroups)
     void *records;          // [rec_cap] sorted row records
-    uint32_t *band;         // [planes*tiles_v] tile rows holding an occupied tile
+    // [planes*ny] per grid row of each plane: non-zero iff the tile kernel
+    // wrote a non-zero cell to it (sparse overwrite only; cleared with the
+    // counts, whose allocation it directly follows).
+    unsigned char *rowlive;
     // FP32 only: per split slot, the count of its tasks done and its 64-bit
     // fixed-point accumulator tile (re then im, tu*tv each); both are zero
     // between calls (cleared at plan creation, then by the bucket's last
@@ -239,8 +242,8 @@ hipError_t launch_phase(const PhaseGeom &p, int precision, const void *gridss, d
 // Image chain after gridding, in place on grid (clobbered) and without
 // scratch: row FFT reading only the tiles whose gridder bucket count
 // (occ_counts, geometry occ_geom; null: dense grid) is non-zero and skipping
-// rows of tile rows without any (flags written to occ_band, planes*tiles_v),
-// column FFT whose last pass is fused with the phase correction and plane sum
+// grid rows whose flag in occ_rows (planes*ny, written by the tile kernel) is
+// zero, column FFT whose last pass is fused with the phase correction and plane sum
 // into the images (overwritten).  Scales by `scale` (1/(Nx*Ny)).
 // Phase geometry and output of one job of a batch group (its planes in the
 // group's plane stack and its images).
@@ -253,8 +256,9 @@ struct PhaseJob {
 };
 hipError_t launch_image_chain(const FftPlan &p, int precision, void *grid, double scale,
                               hipStream_t s, Timing *tm, const GridGeom *occ_geom,
-                              const uint32_t *occ_counts, uint32_t *occ_band, const PhaseGeom &pg,
-                              double *re, double *im, const PhaseJob *jobs = nullptr,
+                              const uint32_t *occ_counts, const unsigned char *occ_rows,
+                              const PhaseGeom &pg, double *re, double *im,
+                              const PhaseJob *jobs = nullptr,
                               int njobs = 0);
 
 }  // namespace rick

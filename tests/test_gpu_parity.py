@@ -553,6 +553,50 @@ def test_chain_band_flags_differ_between_planes(ny, precision):
         assert err <= (FP64_TOL if precision == FP64 else FP32_TOL)
 
 
+@pytest.mark.parametrize("tile", [(128, 32), (32, 64)])
+@pytest.mark.parametrize("precision", [FP64, FP32])
+def test_chain_row_flags_skip_rows_inside_bands(precision, tile):
+    """Image chain on 512 x 1024 x 4 planes (four-step column FFT; the row FFT
+    takes 8 rows per workgroup) with the visibilities in thin v-stripes that
+    start and end inside tile bands, over part of the u range.  The tile
+    kernel flags only the grid rows it writes a non-zero cell to; the row FFT
+    skips the others (whole workgroups, or the rows' loads) and the column
+    passes read them as zero.  The grid starts as NaN, so a row read that
+    should have been skipped, in a tile that was never written, shows as NaN;
+    the second call puts the stripes elsewhere, so flags or rows left from
+    the first call change the image."""
+    rng = np.random.default_rng(47)
+    nx, ny, planes, per = 512, 1024, 4, 900
+    res, wmin, wmax = 0.4 / ny, -200.0, 200.0
+    plan = None
+    for call, centres in enumerate(([0.2013, 0.4470, 0.7302], [0.3108, 0.6219, 0.8825])):
+        uu = np.concatenate([rng.uniform(0.1, 0.6, per) for _ in centres])
+        vv = np.concatenate([c + rng.uniform(0.0, 3.0 / ny, per) for c in centres])
+        ww = rng.uniform(0.02, 0.98, uu.size)
+        n = uu.size
+        vr = rng.normal(size=n).astype(np.float32)
+        vi = rng.normal(size=n).astype(np.float32)
+        wt = rng.uniform(0.5, 1.5, n).astype(np.float32)
+        g = O.wstack(O.KAISERBESSEL, planes, uu, vv, ww, vr, vi, wt, 1, 1, 1.0 / nx,
+                     1.0 / planes, 7, nx, ny, threads=8)
+        gs = O.fft_planes(planes, nx, ny, g, threads=8)
+        ore, oim = O.phase_correction(gs, nx, ny, planes, res, wmin, wmax, threads=8)
+        ref = ore + 1j * oim
+        print(f"case row flags call={call} tile={tile} precision={precision}", flush=True)
+        if plan is None:
+            plan = Plan(nx, ny, planes, kernel=rick_amd.KAISERBESSEL, support=7, dx=1.0 / nx,
+                        dw=1.0 / planes, precision=precision, max_points=n, resolution=res,
+                        wmin=wmin, wmax=wmax, tile=tile)
+            grid = nan_grid(plan)
+        ins = [dev(a) for a in (uu, vv, ww, vr, vi, wt)]
+        re, im, _ = plan.image(*ins, 1, 1, grid=grid)
+        got = host(re) + 1j * host(im)
+        assert np.isfinite(got).all()
+        err = rel_l2(got, ref)
+        print(f"call {call}: rel-L2 {err:.3e}", flush=True)
+        assert err <= (FP64_TOL if precision == FP64 else FP32_TOL)
+
+
 def test_c_program_calls_dropin_chain(tmp_path):
     """A C program (tests/c/dropin_caller.c) linked with -lrick_hip calls
     wstack() -> rick_fft_planes() -> phase_correction() on host arrays; its
