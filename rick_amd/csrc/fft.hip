@@ -156,15 +156,31 @@ struct Lines {
     int nvalid;                 // lines of this workgroup that exist
     int tw_set;                 // twiddle W_N^(n*tw_set) after the last pass (0: none)
     // Optional tile occupancy of the input (rows only): element n of line g
-    // is read only if occ[g * occ_stride + n / occ_tu] is non-zero, else it
-    // is taken as zero (cells of empty gridder tiles are never written).
-    const unsigned char *occ;
-    int occ_stride, occ_tu;
+    // is read only if its tile t = n >> occ_shift is occupied, else it is
+    // taken as zero (cells of empty gridder tiles are never written): bit t
+    // of occ_mask when occ_mask_on (one line of at most 64 tiles), else
+    // occ[g * occ_stride + t] != 0 when occ is set.
+    const unsigned char *occ = nullptr;
+    int occ_stride = 0, occ_shift = 0;
+    bool occ_mask_on = false;
+    unsigned long long occ_mask = 0;
     // Optional row liveness of the input (columns): element n is read only if
     // rowlive[n] is non-zero, else it is zero (grid rows without a non-zero
     // cell are skipped by the row transform).
-    const unsigned char *rowlive;
+    const unsigned char *rowlive = nullptr;
 };
+
+// Whether element n of line g is read from the input (else it is zero).
+__device__ __forceinline__ bool elem_live(const Lines &ln, int g, int n)
+{
+    if (g >= ln.nvalid) return false;
+    if (ln.occ_mask_on) {
+        if (!((ln.occ_mask >> (n >> ln.occ_shift)) & 1ull)) return false;
+    } else if (ln.occ != nullptr && !ln.occ[g * ln.occ_stride + (n >> ln.occ_shift)]) {
+        return false;
+    }
+    return ln.rowlive == nullptr || ln.rowlive[n];
+}
 
 // Butterfly b of a pass with L/R butterflies per line: rows map consecutive
 // b to consecutive j of one line, columns to consecutive lines.
@@ -210,9 +226,7 @@ __device__ __forceinline__ void load_first(C2<T> *vals, int tid, const Lines &ln
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int n = j + r * LR;
-            const bool live =
-                g < ln.nvalid && (ln.occ == nullptr || ln.occ[g * ln.occ_stride + n / ln.occ_tu]) &&
-                (ln.rowlive == nullptr || ln.rowlive[n]);
+            const bool live = elem_live(ln, g, n);
             const size_t at = ln.in_base + (size_t)g * ln.in_line + (size_t)n * ln.in_el;
             vals[i * R + r] = live ? load_stream2(&gin[at]) : C2<T>{T(0), T(0)};
         }
@@ -256,10 +270,7 @@ __device__ __forceinline__ void pass(C2<T> *vals, int tid, const Lines &ln, int 
                     if constexpr (from_lds) {
                         v[r] = lds[lds_at<ROWS, L, G>(g, n)];
                     } else {
-                        const bool live =
-                            g < ln.nvalid &&
-                            (ln.occ == nullptr || ln.occ[g * ln.occ_stride + n / ln.occ_tu]) &&
-                            (ln.rowlive == nullptr || ln.rowlive[n]);
+                        const bool live = elem_live(ln, g, n);
                         v[r] = live ? load_stream2(&gin[ln.in_base + (size_t)g * ln.in_line +
                                                         (size_t)n * ln.in_el])
                                     : C2<T>{T(0), T(0)};
@@ -351,6 +362,7 @@ struct Occupancy {
     const uint32_t *counts;
     const unsigned char *rows;
     int tu, tv, tiles_u, tiles_v, ny;
+    int tu_shift;   // log2(tu): tile widths are powers of two
 };
 
 // Row flags of the grid for the column kernels (null: dense input).
@@ -395,16 +407,31 @@ k_fft_rows(int rows_total, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T>
     ln.in_el = ln.out_el = 1;
     ln.nvalid = (int)min((size_t)G, (size_t)rows_total - row0);
     ln.tw_set = 0;
-    ln.occ = nullptr;
-    ln.rowlive = nullptr;
-    if (oc.rows) {
+    if constexpr (G == 1) {
+        // One row per workgroup: its flag and its tiles' occupancy are
+        // uniform, so no LDS table or barrier is needed.  A row without a
+        // non-zero cell is zero and is neither computed nor written.
+        if (oc.rows && !oc.rows[row0]) return;
+        if (oc.counts && oc.tiles_u <= 64) {
+            // Every wave reads the row's (at most 64) tile counts, one per
+            // lane; the ballot is the occupancy bit mask.
+            const int plane = (int)(row0 / oc.ny), v = (int)(row0 % oc.ny);
+            const int t = threadIdx.x & 63;
+            const bool occd =
+                t < oc.tiles_u &&
+                oc.counts[((size_t)plane * oc.tiles_v + v / oc.tv) * oc.tiles_u + t] != 0;
+            ln.occ_mask = __ballot(occd);
+            ln.occ_mask_on = true;
+            ln.occ_shift = oc.tu_shift;
+        }
+    } else if (oc.rows) {
         // Uniform across the workgroup: none of its rows holds a non-zero
         // cell -> the output rows are zero and are neither computed nor written.
         int live = 0;
         for (int g = threadIdx.x; g < ln.nvalid; g += NT) live |= oc.rows[row0 + g];
         if (!__syncthreads_or(live)) return;
     }
-    if (oc.counts) {
+    if (oc.counts && !ln.occ_mask_on) {
         for (int i = threadIdx.x; i < G * oc.tiles_u && oc.tiles_u <= MAXT; i += NT) {
             const int g = i / oc.tiles_u, t = i % oc.tiles_u;
             const size_t row = row0 + g;
@@ -418,7 +445,7 @@ k_fft_rows(int rows_total, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T>
         __syncthreads();
         ln.occ = occ;
         ln.occ_stride = oc.tiles_u;
-        ln.occ_tu = oc.tu;
+        ln.occ_shift = oc.tu_shift;
     }
     C2<T> vals[E];
     GlobalSink<T> sink{gout, &ln, tw, scale};
@@ -860,10 +887,13 @@ hipError_t launch_image_chain(const FftPlan &p, int precision, void *grid, doubl
     if (tm == nullptr) tm = &off;
     Occupancy oc{};
     if (occ_geom) {
-        if (!occ_counts || p.nx / occ_geom->tu > 256) return hipErrorInvalidValue;
+        if (!occ_counts || p.nx / occ_geom->tu > 256 || occ_geom->tu <= 0 ||
+            (occ_geom->tu & (occ_geom->tu - 1)) != 0)
+            return hipErrorInvalidValue;
         oc.counts = occ_counts;
         oc.rows = occ_rows;
         oc.tu = occ_geom->tu;
+        oc.tu_shift = __builtin_ctz((unsigned)occ_geom->tu);
         oc.tv = occ_geom->tv;
         oc.tiles_u = occ_geom->tiles_u;
         oc.tiles_v = occ_geom->tiles_v;
