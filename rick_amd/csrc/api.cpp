@@ -185,6 +185,8 @@ int rick_plan_create(rick_plan *out, const rick_plan_desc *desc)
     g.inv_dw = exact_inverse(d.dw);
     g.tu = tu;
     g.tv = tv;
+    g.tu_shift = __builtin_ctz((unsigned)tu);
+    g.tv_shift = __builtin_ctz((unsigned)tv);
     g.tiles_u = (g.nx + tu - 1) / tu;
     g.tiles_v = (g.ny + tv - 1) / tv;
     const uint64_t nb = (uint64_t)g.planes * g.tiles_u * g.tiles_v;

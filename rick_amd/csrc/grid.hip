@@ -183,11 +183,12 @@ __device__ __forceinline__ BlockRows block_rows(const GridGeom &g, uint32_t vrow
                      J.planes, J.plane0};
 }
 
-// Buckets of a footprint (support <= tile size, so at most 2 tiles per axis).
+// Buckets of a footprint (support <= tile size, so at most 2 tiles per axis;
+// cell indices are >= 0, so shifts divide by the tile size).
 __device__ __forceinline__ int row_keys(const GridGeom &g, const Foot &f, uint32_t *keys)
 {
-    const int tu0 = f.jmin / g.tu, tu1 = f.jmax / g.tu;
-    const int tv0 = f.kmin / g.tv, tv1 = f.kmax / g.tv;
+    const int tu0 = f.jmin >> g.tu_shift, tu1 = f.jmax >> g.tu_shift;
+    const int tv0 = f.kmin >> g.tv_shift, tv1 = f.kmax >> g.tv_shift;
     const uint32_t base = (uint32_t)f.gw * (uint32_t)g.tiles_v;
     int n = 0;
     for (int tv = tv0; tv <= tv1; ++tv)
@@ -487,9 +488,20 @@ k_scan_down(GridGeom g, const uint32_t *__restrict__ counts,
 // 16-byte read, and every load instruction reads whole lines.  The running
 // sum walks the row's L lanes in element order (shfl_up), so each row's sum is
 // formed in the order of the per-row loop in k_scatter (w-stacking.cu:512-527).
-// Lane l returns the sums of row r0 + l (zero past n).
+// Lane l returns the sums of row r0 + l (zero past n).  widx[c] is the weight
+// index (4(l%L) + c) mod P of this lane's element c in every row
+// (row_weight_index), computed once per thread.
+template <int L>
+__device__ __forceinline__ void row_weight_index(uint32_t P, uint32_t (&widx)[4])
+{
+    const uint32_t q = (threadIdx.x & 63u) % L;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) widx[c] = P == 1u ? 0u : (4u * q + (uint32_t)c) % P;
+}
+
 template <int L>
 __device__ __forceinline__ void wave_row_sums(uint32_t r0, uint32_t n, uint32_t P,
+                                              const uint32_t (&widx)[4],
                                               const float *__restrict__ vr,
                                               const float *__restrict__ vi,
                                               const float *__restrict__ wt, double &out_re,
@@ -513,7 +525,7 @@ __device__ __forceinline__ void wave_row_sums(uint32_t r0, uint32_t n, uint32_t 
             ia = load_stream4(&i4[e]);
             const float *wrow = wt + (size_t)row * P;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) wv[c] = wrow[P == 1 ? 0u : (uint32_t)(4 * q + c) % P];
+            for (int c = 0; c < 4; ++c) wv[c] = wrow[widx[c]];
         }
         const float rr[4] = {ra.x, ra.y, ra.z, ra.w}, ii[4] = {ia.x, ia.y, ia.z, ia.w};
         double run_re = 0.0, run_im = 0.0;
@@ -595,12 +607,14 @@ k_scatter(GridGeom g, uint32_t n_all, uint32_t F, uint32_t P, const double *__re
         }
     }
     double w_re[kRowsPerThread], w_im[kRowsPerThread];
+    uint32_t widx[4] = {0u, 0u, 0u, 0u};
+    if constexpr (VMODE >= 2) row_weight_index<VMODE>(P, widx);
 #pragma unroll
     for (int k = 0; k < kRowsPerThread; ++k) {
         w_re[k] = w_im[k] = 0.0;
         if constexpr (VMODE >= 2)  // whole wave, before any lane leaves
             wave_row_sums<VMODE>(base + k * kScatterThreads + threadIdx.x - (threadIdx.x & 63u),
-                                 n, P, vr, vi, wt, w_re[k], w_im[k]);
+                                 n, P, widx, vr, vi, wt, w_re[k], w_im[k]);
     }
     for (int i = threadIdx.x; i < kHashSlots; i += kScatterThreads) {
         hkey[i] = kEmpty;

@@ -104,7 +104,8 @@ struct GridGeom {
     int support, kl;        // kl = (support-1)/2 (w-stacking.cu:252)
     int kind;               // RICK_GAUSS ...
     double dx, dw;
-    int tu, tv;             // tile size (cells)
+    int tu, tv;             // tile size (cells), powers of two
+    int tu_shift, tv_shift; // log2(tu), log2(tv)
     int tiles_u, tiles_v;   // tiles per axis
     uint32_t nb;            // buckets = planes*tiles_u*tiles_v
     uint32_t rmax;          // rows per task
