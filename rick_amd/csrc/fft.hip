@@ -502,7 +502,7 @@ struct PhaseSink {
     // recurrence, each pixel's current rotation and its per-plane step.
     double nfv[kRecur ? 1 : E];
     double rc[kRecur ? E : 1], rs[kRecur ? E : 1], sc[kRecur ? E : 1], ss[kRecur ? E : 1];
-    const double *nfl;    // FP32: n of the workgroup's pixels in LDS, [n * G + g]
+    const double *nfl;    // FP32: n - 1 of the workgroup's pixels in LDS, [n * G + g]
     int G;
     const double *ltab, *mtab;
     double wterm, dwnorm;
@@ -526,7 +526,7 @@ struct PhaseSink {
         } else if constexpr (sizeof(T) == 8) {
             phase_add_f64(o.x, o.y, wterm, nfv[idx], dwnorm, are[idx], aim[idx]);
         } else {
-            phase_add_f32(o.x, o.y, wterm, nfl[n * G + g], dwnorm, are[idx], aim[idx]);
+            phase_add_f32(o.x, o.y, wterm, nfl[n * G + g], are[idx], aim[idx]);
         }
     }
 };
@@ -590,12 +590,13 @@ k_fft_cols_phase(ColArgs a, int planes_in, const C2<T> *__restrict__ tw,
     constexpr int P = npasses(L, E) - 1, R = radix_of(L, P, E), Ns = ns_of(L, P, E), LR = L / R,
                   BPT = E / R;
     if constexpr (sizeof(T) == 4) {
-        // FP32: n of every pixel once, in LDS (registers are kept for two
-        // waves per SIMD).
+        // FP32: n - 1 of every pixel once, in LDS (registers are kept for
+        // two waves per SIMD).
         for (int e = threadIdx.x; e < G * L; e += NT) {
             const int g = e % G, n = e / G;
-            nfl[e] = g < sink.nvalid ? pixel_n(pg.ltab[c0 + g], pg.mtab[s * vs_mul + n * v_stride])
-                                     : 1.0;
+            nfl[e] = g < sink.nvalid
+                         ? pixel_n(pg.ltab[c0 + g], pg.mtab[s * vs_mul + n * v_stride]) - 1.0
+                         : 0.0;
         }
         sink.nfl = nfl;
         sink.G = G;
@@ -677,7 +678,7 @@ k_fft_cols_phase(ColArgs a, int planes_in, const C2<T> *__restrict__ tw,
             if (g < sink.nvalid) {
                 const size_t pix = (size_t)(s * vs_mul + n * v_stride) * a.nx + (c0 + g);
                 double vre = sink.are[i * R + r], vim = sink.aim[i * R + r];
-                if constexpr (PhaseSink<T, E>::kRecur) {
+                if constexpr (PhaseSink<T, E>::kRecur || sizeof(T) == 4) {
                     if (sink.phase) {
                         const double nf = pixel_n(pg.ltab[c0 + g], pg.mtab[s * vs_mul + n * v_stride]);
                         vre = vre * pg.dwnorm * nf;

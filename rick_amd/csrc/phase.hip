@@ -49,12 +49,17 @@ k_phase(PhaseGeom g, const void *__restrict__ gridss_v, double *__restrict__ ore
             }
         } else {
             const float2 v = reinterpret_cast<const float2 *>(gridss_v)[idx];
-            if (phase) phase_add_f32(v.x, v.y, g.wmin + iw * g.dw, n, g.dwnorm, acc_re, acc_im);
+            if (phase) phase_add_f32(v.x, v.y, g.wmin + iw * g.dw, n - 1.0, acc_re, acc_im);
             else {
                 acc_re += (double)v.x;
                 acc_im += (double)v.y;
             }
         }
+    }
+    if (PREC == RICK_FP32 && phase) {
+        // phase_add_f32 leaves the pixel's factor dwnorm * n to the sum.
+        acc_re = acc_re * g.dwnorm * n;
+        acc_im = acc_im * g.dwnorm * n;
     }
     if (accumulate) {
         acc_re = ore[pix] + acc_re;

@@ -78,20 +78,19 @@ __device__ __forceinline__ void phase_add_f64(double p, double q, double wterm, 
     aim += (p * sn + q * cs) * dwnorm * nf;
 }
 
-// FP32 plane data: the phase argument and its reduction to one turn stay
-// FP64; the rotation is FP32 (the hardware sine/cosine of a fraction of a
-// turn in [-1/2, 1/2], absolute error a few 1e-7), the accumulation FP64.
-__device__ __forceinline__ void phase_add_f32(float p, float q, double wterm, double nf,
-                                              double dwnorm, double &are, double &aim)
+// FP32 plane data: the phase in turns, wterm * (n - 1) = arg / (2 pi), and
+// its reduction to one turn stay FP64; the rotation is FP32 (the hardware
+// sine/cosine of a fraction of a turn in [-1/2, 1/2], absolute error a few
+// 1e-7), the accumulation FP64.  The pixel's factor dwnorm * n is applied
+// once to the plane sum by the caller.
+__device__ __forceinline__ void phase_add_f32(float p, float q, double wterm, double nm1,
+                                              double &are, double &aim)
 {
-#pragma clang fp contract(off)
-    const double arg = 2.0 * kRefPi * wterm * (nf - 1.0);
-    const double turns = arg * (1.0 / (2.0 * 3.141592653589793));
+    const double turns = wterm * nm1;
     const float f = (float)(turns - rint(turns));
     const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
-    const float fre = (p * cs - q * sn), fim = (p * sn + q * cs);
-    are += (double)fre * dwnorm * nf;
-    aim += (double)fim * dwnorm * nf;
+    are += (double)(p * cs - q * sn);
+    aim += (double)(p * sn + q * cs);
 }
 
 }  // namespace rick
