@@ -845,7 +845,14 @@ hipError_t image_chain_impl(const FftPlan &p, void *grid_v, double scale_d, hipS
         tm->end(RICK_STAGE_PHASE, s);
         return e;
     }
-    const int N1 = 64, N2 = ny / 64;
+    // ny = N1 * N2.  FP64 at ny = 8192 splits 128 x 64: the fused pass B
+    // repeats its N2-point transform for every plane, so it gets the shorter
+    // lines (one LDS exchange at 64 points, two at 128) while pass A, once
+    // per plane, takes the 128-point ones (JVLA-like chain, 8192^2 x 64: pass
+    // B 16.7 -> 13.9 ms, pass A 17.3 -> 18.0 ms).  FP32 keeps 64 x 128: its
+    // 128-point pass A at 8 values per thread (three radix passes) cost more
+    // than pass B saved (9.3 -> 11.2 ms against 8.1 -> 7.2 ms).
+    const int N1 = (sizeof(T) == 8 && ny > 4096) ? ny / 64 : 64, N2 = ny / N1;
     // A: rows n2 + N2*n1 -> n2 + N2*k1 (in place), twiddle W_N^(k1*n2).
     const ColArgs At{nx, ny, 1, N2, 1, N2, 1, N2};
     // B: rows N2*k1 + n2 -> X[k1 + N1*k2], image row k1 + N1*k2.
