@@ -395,9 +395,10 @@ k_fft_rows(int rows_total, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T>
     constexpr int MAXT = 256;  // tiles per row held in LDS (tile width >= 32, L <= 8192)
     // FP64 exchanges split into real and imaginary halves: one T per element
     // in LDS, so twice the workgroups fit per CU (1.61 -> 1.35 ms on 32
-    // planes of 4096^2).  FP32 keeps whole exchanges, which measured as fast
-    // on large grids and faster on small ones.
-    constexpr bool kSplit = sizeof(T) == 8;
+    // planes of 4096^2).  FP32 splits rows of 4096 and more too (0.658 ->
+    // 0.632 ms on the same planes) and keeps whole exchanges on shorter rows,
+    // where splitting measured about 2 % slower on the batched jobs.
+    constexpr bool kSplit = sizeof(T) == 8 || L >= 4096;
     __shared__ C2<T> lds[kSplit ? (lds_elems(L, true) + 1) / 2 : lds_elems(L, true)];
     __shared__ unsigned char occ[oc_bytes(L)];
     Lines ln;
