@@ -468,7 +468,9 @@ __global__ void __launch_bounds__(G * L / E)
 k_fft_cols(ColArgs a, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gout, T scale,
            ColOcc co)
 {
-    constexpr bool kSplit = sizeof(T) == 8;   // as in k_fft_rows
+    // FP64 only: an FP32 workgroup here (up to 1024 threads) is already at
+    // two per CU with whole exchanges, so halving its LDS adds none.
+    constexpr bool kSplit = sizeof(T) == 8;
     __shared__ C2<T> lds[kSplit ? G * L / 2 : G * L];
     __shared__ unsigned char rl[L];
     const int c0 = blockIdx.x * G;
