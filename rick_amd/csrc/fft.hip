@@ -385,7 +385,10 @@ __device__ __forceinline__ void col_rowlive(const ColOcc &co, int plane, int row
 }
 
 // Rows: workgroup w handles rows w*G .. w*G+G-1 of the planes*ny rows.
-template <typename T, int L, int E>
+// MASK (one row per workgroup, bucket counts of at most 64 tiles per row):
+// the row's tile occupancy is a bit mask in scalar registers, so the test of
+// each loaded element compiles to a shift and a select.
+template <typename T, int L, int E, bool MASK>
 __global__ void __launch_bounds__(lines_per_wg(L) * L / E)
 k_fft_rows(int rows_total, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T> *gout, T scale,
            Occupancy oc)
@@ -406,25 +409,25 @@ k_fft_rows(int rows_total, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T>
     ln.in_base = ln.out_base = row0 * L;
     ln.in_line = ln.out_line = L;
     ln.in_el = ln.out_el = 1;
-    ln.nvalid = (int)min((size_t)G, (size_t)rows_total - row0);
+    // One row per workgroup: the grid has exactly rows_total workgroups.
+    ln.nvalid = G == 1 ? 1 : (int)min((size_t)G, (size_t)rows_total - row0);
     ln.tw_set = 0;
-    if constexpr (G == 1) {
-        // One row per workgroup: its flag and its tiles' occupancy are
-        // uniform, so no LDS table or barrier is needed.  A row without a
-        // non-zero cell is zero and is neither computed nor written.
+    if constexpr (MASK) {
+        static_assert(G == 1, "the occupancy mask covers one row");
+        // The row's flag and its tiles' occupancy are uniform, so no LDS
+        // table or barrier is needed.  A row without a non-zero cell is zero
+        // and is neither computed nor written.
         if (oc.rows && !oc.rows[row0]) return;
-        if (oc.counts && oc.tiles_u <= 64) {
-            // Every wave reads the row's (at most 64) tile counts, one per
-            // lane; the ballot is the occupancy bit mask.
-            const int plane = (int)(row0 / oc.ny), v = (int)(row0 % oc.ny);
-            const int t = threadIdx.x & 63;
-            const bool occd =
-                t < oc.tiles_u &&
-                oc.counts[((size_t)plane * oc.tiles_v + v / oc.tv) * oc.tiles_u + t] != 0;
-            ln.occ_mask = __ballot(occd);
-            ln.occ_mask_on = true;
-            ln.occ_shift = oc.tu_shift;
-        }
+        // Every wave reads the row's (at most 64) tile counts, one per lane;
+        // the ballot is the occupancy bit mask.
+        const int plane = (int)(row0 / oc.ny), v = (int)(row0 % oc.ny);
+        const int t = threadIdx.x & 63;
+        const bool occd =
+            t < oc.tiles_u &&
+            oc.counts[((size_t)plane * oc.tiles_v + v / oc.tv) * oc.tiles_u + t] != 0;
+        ln.occ_mask = __ballot(occd);
+        ln.occ_mask_on = true;
+        ln.occ_shift = oc.tu_shift;
     } else if (oc.rows) {
         // Uniform across the workgroup: none of its rows holds a non-zero
         // cell -> the output rows are zero and are neither computed nor written.
@@ -432,7 +435,7 @@ k_fft_rows(int rows_total, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T>
         for (int g = threadIdx.x; g < ln.nvalid; g += NT) live |= oc.rows[row0 + g];
         if (!__syncthreads_or(live)) return;
     }
-    if (oc.counts && !ln.occ_mask_on) {
+    if (!MASK && oc.counts) {
         for (int i = threadIdx.x; i < G * oc.tiles_u && oc.tiles_u <= MAXT; i += NT) {
             const int g = i / oc.tiles_u, t = i % oc.tiles_u;
             const size_t row = row0 + g;
@@ -704,7 +707,14 @@ hipError_t launch_rows(int L, int rows_total, const C2<T> *tw, const C2<T> *in, 
     case LL: {                                                                             \
         constexpr int G = lines_per_wg(LL);                                                \
         constexpr int E = row_values<T>();                                                 \
-        hipLaunchKernelGGL((k_fft_rows<T, LL, E>), dim3((rows_total + G - 1) / G),         \
+        if constexpr (G == 1) {                                                            \
+            if (oc.counts && oc.tiles_u <= 64) {                                           \
+                hipLaunchKernelGGL((k_fft_rows<T, LL, E, true>), dim3(rows_total),         \
+                                   dim3(LL / E), 0, s, rows_total, tw, in, out, scale, oc); \
+                return hipGetLastError();                                                  \
+            }                                                                              \
+        }                                                                                  \
+        hipLaunchKernelGGL((k_fft_rows<T, LL, E, false>), dim3((rows_total + G - 1) / G),  \
                            dim3(G * LL / E), 0, s, rows_total, tw, in, out, scale, oc);    \
         return hipGetLastError();                                                          \
     }
