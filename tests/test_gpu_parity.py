@@ -597,6 +597,53 @@ def test_chain_row_flags_skip_rows_inside_bands(precision, tile):
         assert err <= (FP64_TOL if precision == FP64 else FP32_TOL)
 
 
+_ONE_ROW_CASE = {}
+
+
+def one_row_case():
+    """4096^2 x 2 planes, visibilities in two v-bands over the middle of the
+    u range, and the oracle's image of them (computed once)."""
+    if not _ONE_ROW_CASE:
+        rng = np.random.default_rng(5)
+        nx = ny = 4096
+        planes, n = 2, 20000
+        uu = rng.uniform(0.3, 0.7, n)
+        vv = np.concatenate([rng.uniform(0.30, 0.36, n // 2), rng.uniform(0.55, 0.58, n - n // 2)])
+        ww = rng.uniform(0.02, 0.98, n)
+        vr = rng.normal(size=n).astype(np.float32)
+        vi = rng.normal(size=n).astype(np.float32)
+        wt = rng.uniform(0.5, 1.5, n).astype(np.float32)
+        res, wmin, wmax = 0.4 / ny, -200.0, 200.0
+        g = O.wstack(O.KAISERBESSEL, planes, uu, vv, ww, vr, vi, wt, 1, 1, 1.0 / nx,
+                     1.0 / planes, 7, nx, ny, threads=8)
+        gs = O.fft_planes(planes, nx, ny, g, threads=8)
+        ore, oim = O.phase_correction(gs, nx, ny, planes, res, wmin, wmax, threads=8)
+        _ONE_ROW_CASE.update(nx=nx, ny=ny, planes=planes, n=n, res=res, wmin=wmin, wmax=wmax,
+                             ins=(uu, vv, ww, vr, vi, wt), ref=ore + 1j * oim)
+    return _ONE_ROW_CASE
+
+
+@pytest.mark.parametrize("tile", [(64, 64), (32, 32)])
+@pytest.mark.parametrize("precision", [FP64, FP32])
+def test_chain_one_row_workgroups_both_occupancy_paths(precision, tile):
+    """Image chain at 4096^2, where the row FFT takes one row per workgroup.
+    With 64-wide tiles a row has 64 tiles and the row kernel's occupancy-mask
+    instance runs; with 32-wide tiles a row has 128 and the instance with the
+    LDS occupancy table runs.  The grid starts as NaN, so a load from a tile
+    that was never written shows in the image."""
+    c = one_row_case()
+    print(f"case one-row chain tile={tile} precision={precision}", flush=True)
+    plan = Plan(c["nx"], c["ny"], c["planes"], kernel=rick_amd.KAISERBESSEL, support=7,
+                dx=1.0 / c["nx"], dw=1.0 / c["planes"], precision=precision, max_points=c["n"],
+                resolution=c["res"], wmin=c["wmin"], wmax=c["wmax"], tile=tile)
+    re, im, _ = plan.image(*[dev(a) for a in c["ins"]], 1, 1, grid=nan_grid(plan))
+    got = host(re) + 1j * host(im)
+    assert np.isfinite(got).all()
+    err = rel_l2(got, c["ref"])
+    print(f"one-row chain tile={tile} precision={precision}: rel-L2 {err:.3e}", flush=True)
+    assert err <= (FP64_TOL if precision == FP64 else FP32_TOL)
+
+
 def test_c_program_calls_dropin_chain(tmp_path):
     """A C program (tests/c/dropin_caller.c) linked with -lrick_hip calls
     wstack() -> rick_fft_planes() -> phase_correction() on host arrays; its
