@@ -361,8 +361,8 @@ constexpr int oc_bytes(int L) { return lines_per_wg(L) * (L / 32 > 0 ? L / 32 : 
 struct Occupancy {
     const uint32_t *counts;
     const unsigned char *rows;
-    int tu, tv, tiles_u, tiles_v, ny;
-    int tu_shift;   // log2(tu): tile widths are powers of two
+    int tv, tiles_u, tiles_v, ny;
+    int tu_shift;   // log2 of the tile width (a power of two)
 };
 
 // Row flags of the grid for the column kernels (null: dense input).
@@ -913,7 +913,6 @@ hipError_t launch_image_chain(const FftPlan &p, int precision, void *grid, doubl
             return hipErrorInvalidValue;
         oc.counts = occ_counts;
         oc.rows = occ_rows;
-        oc.tu = occ_geom->tu;
         oc.tu_shift = __builtin_ctz((unsigned)occ_geom->tu);
         oc.tv = occ_geom->tv;
         oc.tiles_u = occ_geom->tiles_u;
