@@ -398,10 +398,11 @@ k_fft_rows(int rows_total, const C2<T> *__restrict__ tw, const C2<T> *gin, C2<T>
     constexpr int MAXT = 256;  // tiles per row held in LDS (tile width >= 32, L <= 8192)
     // FP64 exchanges split into real and imaginary halves: one T per element
     // in LDS, so twice the workgroups fit per CU (1.61 -> 1.35 ms on 32
-    // planes of 4096^2).  FP32 splits rows of 4096 and more too (0.658 ->
-    // 0.632 ms on the same planes) and keeps whole exchanges on shorter rows,
-    // where splitting measured about 2 % slower on the batched jobs.
-    constexpr bool kSplit = sizeof(T) == 8 || L >= 4096;
+    // planes of 4096^2).  FP32 splits 4096-point rows too (0.658 -> 0.632 ms
+    // on the same planes) and keeps whole exchanges elsewhere: splitting
+    // measured slower on 8192-point rows (64 planes of 8192^2: 3.9 -> 4.2 ms)
+    // and about 2 % slower on the batched jobs' shorter rows.
+    constexpr bool kSplit = sizeof(T) == 8 || L == 4096;
     __shared__ C2<T> lds[kSplit ? (lds_elems(L, true) + 1) / 2 : lds_elems(L, true)];
     __shared__ unsigned char occ[oc_bytes(L)];
     Lines ln;
