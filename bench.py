@@ -349,8 +349,8 @@ def batch_leg(torch, steps=3):
     support 9), against the same jobs imaged one after another in FP64:
     each job's chain on four streams captured into one HIP graph, and the
     fused batch (one launch per stage per group of jobs sharing grid size and
-    kernel) captured likewise.  Every job's image is compared with its
-    sequential FP64 image."""
+    kernel, the groups dealt to four streams) captured likewise.  Every job's
+    image is compared with its sequential FP64 image."""
     from rick_amd import FP32, FP64, batch, synth
     cfgs = batch.job_configs(96)
     datasets = [synth.make_dataset(c, device="cuda:0") for c in cfgs]
@@ -387,6 +387,7 @@ def batch_leg(torch, steps=3):
         t_graph = timed(fb.replay, steps, torch) * 1e-3
         errs = [float(np.linalg.norm(a - r) / np.linalg.norm(r)) for a, r in zip(fb.images(), ref)]
         out[name] = {"graph_ms": t_graph * 1e3, "groups": len(fb.groups),
+                     "streams": len(fb.streams),
                      "images_per_s": len(cfgs) / t_graph, "vis_per_s": total_vis / t_graph,
                      "max_rel_l2_vs_seq_fp64": max(errs)}
         del fb
